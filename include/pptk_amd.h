@@ -1,0 +1,224 @@
+/*
+ * pptk_amd.h -- C ABI of the MI355X (gfx950) k-d tree and normal-estimation
+ * library `libpptk_amd.so`.
+ *
+ * Plain C: no HIP, torch or numpy types cross this boundary. Buffers are raw
+ * pointers plus sizes; a `pptk_mem` argument says whether a buffer lives in
+ * host memory (the library copies it) or in device memory (used in place).
+ * Every entry point returns a `pptk_status`; on failure `pptk_last_error()`
+ * returns a message for the calling thread. Status codes map one-to-one onto
+ * the Python exceptions raised by pptk's CPython extensions:
+ *   PPTK_E_VALUE -> ValueError, PPTK_E_TYPE -> TypeError,
+ *   PPTK_E_RUNTIME / PPTK_E_DEVICE -> RuntimeError, PPTK_E_MEMORY ->
+ *   MemoryError, PPTK_E_NOTIMPL -> NotImplementedError.
+ *
+ * Results contract (identical to pptk's CPU k-d tree on the same input):
+ *   - squared distance d = ((0 + t0*t0) + t1*t1) + t2*t2 with
+ *     t_i = (DistT)p_i - (DistT)q_i, evaluated in DistT without FMA
+ *     contraction (pptk/kdtree/src/kdtree-impl.h:331-339). DistT is double for
+ *     float64 points and float for float32 and every integer dtype
+ *     (pptk/kdtree/src/accumulator.h:12-27).
+ *   - a neighbour qualifies iff d < r*r with r = (DistT)dmax
+ *     (kdtree-impl.h:365, 881).
+ *   - kNN rows hold the k smallest (d, original index) pairs in ascending
+ *     order; ties in d are broken by the smaller original point index.
+ *   - self queries exclude the query point itself (kdtree-impl.h:931-947).
+ */
+#ifndef PPTK_AMD_H_
+#define PPTK_AMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  PPTK_OK = 0,
+  PPTK_E_VALUE = 1,
+  PPTK_E_TYPE = 2,
+  PPTK_E_RUNTIME = 3,
+  PPTK_E_MEMORY = 4,
+  PPTK_E_DEVICE = 5,
+  PPTK_E_NOTIMPL = 6
+} pptk_status;
+
+/* Point / query element types: the ten dtypes pptk's kd-tree dispatches on
+ * (pptk/kdtree/kdtree_wrapper.cpp:15-41). */
+typedef enum {
+  PPTK_F32 = 0,
+  PPTK_F64 = 1,
+  PPTK_I8 = 2,
+  PPTK_I16 = 3,
+  PPTK_I32 = 4,
+  PPTK_I64 = 5,
+  PPTK_U8 = 6,
+  PPTK_U16 = 7,
+  PPTK_U32 = 8,
+  PPTK_U64 = 9
+} pptk_dtype;
+
+typedef enum { PPTK_MEM_HOST = 0, PPTK_MEM_DEVICE = 1 } pptk_mem;
+
+/* Opaque device-resident k-d tree. Owns its device buffers. Immutable after
+ * build, so concurrent queries on one tree are safe. */
+typedef struct pptk_tree pptk_tree;
+
+/* Build parameters. Replaces pointkd::BuildParams (pptk/kdtree/src/kdtree.h:29-42).
+ * max_leaf_size: points per leaf bucket; <= 0 picks the tuned default,
+ *   otherwise clamped to [1, 64]. The bucket size changes speed only, never
+ *   results.
+ * num_proc, empty_split: accepted for signature parity and ignored (the GPU
+ *   tree uses tight per-node boxes, which trim empty space without
+ *   empty-split nodes). */
+typedef struct {
+  int max_leaf_size;
+  int num_proc;
+  double empty_split;
+} pptk_build_params;
+
+/* Query set for pptk_knn / pptk_rnear_*. Replaces the three query forms of
+ * kdtree._query (kdtree_wrapper.cpp:161-248):
+ *   PPTK_Q_ALL     : original indices 0 .. num_valid-1 (queries=None,
+ *                    kdtree_wrapper.cpp:176-179), self queries;
+ *   PPTK_Q_INDICES : m original indices (already range-checked and made
+ *                    non-negative by the caller), self queries
+ *                    (kdtree_wrapper.cpp:237-239);
+ *   PPTK_Q_POINTS  : m x dim row-major points of `dtype`
+ *                    (kdtree_wrapper.cpp:186-236). With mem == DEVICE the
+ *                    dtype must equal the tree's distance type. */
+typedef enum { PPTK_Q_ALL = 0, PPTK_Q_INDICES = 1, PPTK_Q_POINTS = 2 } pptk_query_kind;
+
+typedef struct {
+  pptk_query_kind kind;
+  int64_t m;
+  const int32_t* indices;
+  const void* points;
+  pptk_dtype dtype;
+  pptk_mem mem;
+} pptk_queries;
+
+typedef struct {
+  int dim;
+  pptk_dtype dtype;      /* dtype the tree was built from */
+  pptk_dtype dist_dtype; /* PPTK_F32 or PPTK_F64 */
+  int64_t num_input;     /* rows passed to build */
+  int64_t num_valid;     /* rows without NaN/inf */
+  int leaf_size;
+  int64_t num_leaves;
+  int64_t leaf_slots;    /* power of two >= num_leaves */
+  int depth;             /* log2(leaf_slots) */
+  double bbox_lo[4];
+  double bbox_hi[4];
+  int64_t device_bytes;  /* device memory held by the tree */
+} pptk_tree_info;
+
+/* Message of the last failure on this thread ("" if none). */
+const char* pptk_last_error(void);
+
+/* Number of visible HIP devices (0 on a machine without a GPU). */
+int pptk_device_count(void);
+
+/* Release device memory cached by the library's allocator. */
+pptk_status pptk_trim(void);
+
+/* Build a tree. Replaces kdtree._build -> KdTree ctor -> impl::BuildTree
+ * (kdtree_wrapper.cpp:281-304, kdtree-impl.h:773-818): NaN/inf rows are
+ * excluded (kdtree-impl.h:53-75), the tree keeps its own reordered copy.
+ * pts: n x dim row-major, contiguous. dim in {2,3,4}. stream: hipStream_t or
+ * NULL for the library stream. Returns after the tree is complete. */
+pptk_status pptk_tree_build(const void* pts, int64_t n, int dim, pptk_dtype dtype,
+                            const pptk_build_params* params, pptk_mem mem,
+                            void* stream, pptk_tree** out);
+
+/* Free a tree (the capsule destructor, kdtree_wrapper.cpp:57-74). NULL is a no-op. */
+void pptk_tree_free(pptk_tree* tree);
+
+/* 1 if `p` is a live tree handle produced by this library, else 0. */
+int pptk_tree_check(const void* p);
+
+pptk_status pptk_tree_get_info(const pptk_tree* tree, pptk_tree_info* out);
+
+/* Copy the tree's arrays out, mirroring KdTree::points()/indices()/
+ * reverse_indices() (kdtree.h:231-234). Any pointer may be NULL.
+ *   points  : num_valid x dim in tree order, dist dtype
+ *   reverse : num_valid int32, tree position -> original index
+ *   forward : num_input int32, original index -> tree position or -1
+ *   boxes   : (2*leaf_slots-1) x 2 x dim dist dtype; node i = [lo(dim), hi(dim)],
+ *             children of i are 2i+1 and 2i+2, leaf j is node leaf_slots-1+j
+ *             and holds tree positions [j*leaf_size, min(num_valid,(j+1)*leaf_size)). */
+pptk_status pptk_tree_export(const pptk_tree* tree, void* points, int32_t* reverse,
+                             int32_t* forward, void* boxes, pptk_mem mem);
+
+/* k nearest neighbours with distance < dmax. Replaces
+ * KdTree::KNearestNeighbors / KNearestNeighborsSelf batch calls
+ * (kdtree-impl.h:873-976) behind kdtree._query(k>0).
+ * dmax: pass the value already rounded through float32 as kdtree._query does
+ * (pptk/include/python_util.h:393-400); INFINITY for no bound.
+ * out_idx: m x k int32 (row-major), original indices ascending by (d, index),
+ * -1 padded. out_cnt: m int32 neighbour counts. out_dist: NULL or m x k of the
+ * tree's dist dtype (squared distances, +inf padded). For PPTK_Q_ALL, m is
+ * the tree's num_valid. Output buffers live in `out_mem`. */
+pptk_status pptk_knn(const pptk_tree* tree, const pptk_queries* q, int k, double dmax,
+                     int32_t* out_idx, int32_t* out_cnt, void* out_dist,
+                     pptk_mem out_mem, void* stream);
+
+/* r-near neighbours (d < dmax*dmax), two phases. Replaces
+ * KdTree::RNearNeighbors / RNearNeighborsSelf (kdtree-impl.h:978-1069)
+ * behind kdtree._query(k=None). Phase 1 writes CSR offsets (m+1 int64,
+ * offsets[m] = total) and the total into *out_total (host). Phase 2 writes
+ * the neighbour lists (original indices) given those offsets. Within a list,
+ * neighbours appear in tree order; compare lists as sets. */
+pptk_status pptk_rnear_count(const pptk_tree* tree, const pptk_queries* q, double dmax,
+                             int64_t* out_offsets, int64_t* out_total, pptk_mem out_mem,
+                             void* stream);
+pptk_status pptk_rnear_fill(const pptk_tree* tree, const pptk_queries* q, double dmax,
+                            const int64_t* offsets, int32_t* out_idx, pptk_mem out_mem,
+                            void* stream);
+
+/* Normal estimation precision modes. */
+typedef enum {
+  PPTK_PREC_AUTO = 0,  /* FP64 for float64 points, MIXED for float32 points */
+  PPTK_PREC_MIXED = 1, /* FP32 covariance centred on the query point with
+                          compensated (double-float) sums; FP32 eigensolve;
+                          FP64 eigensolve where (l1-l0)/l2 < 1e-3 */
+  PPTK_PREC_FP64 = 2   /* FP64 covariance and eigensolve */
+} pptk_precision;
+
+/* PCA normals on a built 3-D float tree. Replaces the per-point loop of
+ * estimate_normals<T> (pptk/processing/estimate_normals/estimate_normals.cpp:
+ * 64-111): neighbourhood = k nearest including the point itself with d < r*r
+ * (or, for k == -1, all points with d < r*r), C = X^T X / n of the centred
+ * neighbourhood, eigen-decomposition with eigenvalues ascending.
+ * rows: NULL -> one row per input point (original order, num_input rows);
+ *   else m original indices (the subsample, already range-checked).
+ * num_eigen 1: out_evecs m x 3 (eigenvector of the smallest eigenvalue),
+ *   out_evals m; num_eigen 3: out_evecs m x 3 x 3 with [i,j,:] the j-th
+ *   eigenvector, out_evals m x 3. Output element type = the tree's dtype
+ *   (float32 or float64). Eigenvector sign: the component of largest
+ *   magnitude is positive. Rows whose point is NaN/inf or has an empty
+ *   neighbourhood get NaN vectors/values and size 0.
+ * out_evals / out_sizes may be NULL. out_num_fp64: NULL or receives the
+ * number of rows that took the FP64 eigensolve. */
+pptk_status pptk_normals(const pptk_tree* tree, int k, double r, const int32_t* rows,
+                         int64_t m, int num_eigen, pptk_precision precision,
+                         void* out_evecs, void* out_evals, int32_t* out_sizes,
+                         int64_t* out_num_fp64, pptk_mem mem, void* stream);
+
+/* Building blocks exported for testing and for callers that need them. */
+
+/* Stable ascending sort of (key, value) pairs by the key bits
+ * [0, end_bit), in place. Buffers in `mem`. */
+pptk_status pptk_sort_pairs_u64(uint64_t* keys, uint32_t* vals, int64_t n, int end_bit,
+                                pptk_mem mem);
+
+/* Exclusive prefix sum of n int32 counts into n+1 int64 offsets. */
+pptk_status pptk_exclusive_scan_i32(const int32_t* in, int64_t* out, int64_t n,
+                                    pptk_mem mem);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PPTK_AMD_H_ */

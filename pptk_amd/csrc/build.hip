@@ -1,0 +1,622 @@
+// GPU k-d tree build.
+//
+// Replaces impl::BuildTree (pptk/kdtree/src/kdtree-impl.h:773-818). The
+// reference splits each node at the midpoint of its widest side and recurses
+// (kdtree-impl.h:175-329); result sets do not depend on the tree shape
+// (pruning bounds are exact lower/upper bounds), so the GPU tree is laid out
+// for 64-wide traversal instead:
+//   K0 ingest_reduce   validity (no NaN/inf, kdtree-impl.h:53-75) + per-tile
+//                      counts and bounding boxes
+//   K0 ingest_finalize tile offsets, global box, Morton scale
+//   K1 morton          stable compaction of valid rows + Morton code
+//   K2 radix sort      (sort.hip) on the 63-bit code
+//   K3 gather          tree-order point records, reverse/forward maps
+//                      (kdtree-impl.h:803-817)
+//   K3 leaf/level boxes buckets of `leaf` consecutive points; implicit
+//                      complete binary tree (children 2i+1, 2i+2) whose node
+//                      boxes are the tight union of their buckets, built
+//                      level by level from the leaves up.
+#include <algorithm>
+#include <mutex>
+#include <set>
+
+#include "sort.hpp"
+#include "tree.hpp"
+
+namespace pk {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kPer = 16;                 // consecutive points per thread
+constexpr int kTile = kBlock * kPer;     // 4096
+
+template <int DIM> struct MortonBits { static constexpr int value = DIM == 2 ? 31 : DIM == 3 ? 21 : 15; };
+
+template <typename D>
+struct BuildInfo {
+  uint32_t n_valid;
+  uint32_t pad;
+  D lo[4];
+  D hi[4];
+  D scale[4];
+};
+
+template <typename T>
+__device__ __forceinline__ bool elem_finite(T v) {
+  if constexpr (std::is_floating_point<T>::value) return finite_val(v);
+  else return true;
+}
+
+template <typename D>
+__device__ __forceinline__ D wave_min(D x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) { D y = __shfl_xor(x, o); x = y < x ? y : x; }
+  return x;
+}
+template <typename D>
+__device__ __forceinline__ D wave_max(D x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) { D y = __shfl_xor(x, o); x = y > x ? y : x; }
+  return x;
+}
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+  return x;
+}
+
+template <typename T, typename D, int DIM>
+__global__ __launch_bounds__(kBlock) void ingest_reduce_kernel(const T* __restrict__ pts, int64_t n,
+                                                               uint32_t* __restrict__ blk_cnt,
+                                                               D* __restrict__ blk_lo,
+                                                               D* __restrict__ blk_hi) {
+  const int64_t first = (int64_t)blockIdx.x * kTile + (int64_t)threadIdx.x * kPer;
+  uint32_t cnt = 0;
+  D lo[DIM], hi[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) { lo[d] = pos_inf<D>(); hi[d] = -pos_inf<D>(); }
+  for (int e = 0; e < kPer; ++e) {
+    const int64_t i = first + e;
+    if (i >= n) break;
+    T v[DIM];
+    bool ok = true;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) { v[d] = pts[i * DIM + d]; ok = ok && elem_finite(v[d]); }
+    if (ok) {
+      ++cnt;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        const D c = (D)v[d];
+        lo[d] = c < lo[d] ? c : lo[d];
+        hi[d] = c > hi[d] ? c : hi[d];
+      }
+    }
+  }
+  cnt = wave_sum_u32(cnt);
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) { lo[d] = wave_min(lo[d]); hi[d] = wave_max(hi[d]); }
+  __shared__ uint32_t s_cnt[kBlock / 64];
+  __shared__ D s_lo[kBlock / 64][DIM];
+  __shared__ D s_hi[kBlock / 64][DIM];
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    s_cnt[w] = cnt;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) { s_lo[w][d] = lo[d]; s_hi[w][d] = hi[d]; }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t c = 0;
+    D l[DIM], h[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) { l[d] = pos_inf<D>(); h[d] = -pos_inf<D>(); }
+    for (int ww = 0; ww < kBlock / 64; ++ww) {
+      c += s_cnt[ww];
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        l[d] = s_lo[ww][d] < l[d] ? s_lo[ww][d] : l[d];
+        h[d] = s_hi[ww][d] > h[d] ? s_hi[ww][d] : h[d];
+      }
+    }
+    blk_cnt[blockIdx.x] = c;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      blk_lo[(int64_t)blockIdx.x * DIM + d] = l[d];
+      blk_hi[(int64_t)blockIdx.x * DIM + d] = h[d];
+    }
+  }
+}
+
+// One block: exclusive scan of tile counts into tile offsets, global box,
+// Morton quantisation scale.
+template <typename D, int DIM>
+__global__ __launch_bounds__(kBlock) void ingest_finalize_kernel(const uint32_t* __restrict__ blk_cnt,
+                                                                 const D* __restrict__ blk_lo,
+                                                                 const D* __restrict__ blk_hi,
+                                                                 int64_t nblk,
+                                                                 uint32_t* __restrict__ blk_off,
+                                                                 BuildInfo<D>* __restrict__ info) {
+  __shared__ uint32_t s_wsum[kBlock / 64];
+  __shared__ D s_lo[kBlock / 64][DIM];
+  __shared__ D s_hi[kBlock / 64][DIM];
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  uint32_t carry = 0;
+  D lo[DIM], hi[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) { lo[d] = pos_inf<D>(); hi[d] = -pos_inf<D>(); }
+  for (int64_t b0 = 0; b0 < nblk; b0 += kBlock) {
+    const int64_t b = b0 + threadIdx.x;
+    const uint32_t c = b < nblk ? blk_cnt[b] : 0u;
+    if (b < nblk) {
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        const D l = blk_lo[b * DIM + d], h = blk_hi[b * DIM + d];
+        lo[d] = l < lo[d] ? l : lo[d];
+        hi[d] = h > hi[d] ? h : hi[d];
+      }
+    }
+    // block exclusive scan of c
+    uint32_t incl = c;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(incl, o);
+      if (lane >= o) incl += y;
+    }
+    if (lane == 63) s_wsum[w] = incl;
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < kBlock / 64; ++i) {
+      if (i < w) off += s_wsum[i];
+      tot += s_wsum[i];
+    }
+    if (b < nblk) blk_off[b] = carry + off + incl - c;
+    carry += tot;
+    __syncthreads();
+  }
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) { lo[d] = wave_min(lo[d]); hi[d] = wave_max(hi[d]); }
+  if (lane == 0) {
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) { s_lo[w][d] = lo[d]; s_hi[w][d] = hi[d]; }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    info->n_valid = carry;
+    info->pad = 0;
+    const D maxq = (D)((1u << MortonBits<DIM>::value) - 1u);
+    for (int d = 0; d < 4; ++d) {
+      D l = pos_inf<D>(), h = -pos_inf<D>();
+      if (d < DIM) {
+        for (int ww = 0; ww < kBlock / 64; ++ww) {
+          l = s_lo[ww][d] < l ? s_lo[ww][d] : l;
+          h = s_hi[ww][d] > h ? s_hi[ww][d] : h;
+        }
+      } else {
+        l = h = D(0);
+      }
+      info->lo[d] = l;
+      info->hi[d] = h;
+      const D wdt = h - l;
+      info->scale[d] = (d < DIM && wdt > D(0) && finite_val(wdt)) ? maxq / wdt : D(0);
+    }
+  }
+}
+
+__device__ __forceinline__ uint64_t spread3_21(uint32_t a) {
+  uint64_t x = a & 0x1fffffull;
+  x = (x | (x << 32)) & 0x1f00000000ffffull;
+  x = (x | (x << 16)) & 0x1f0000ff0000ffull;
+  x = (x | (x << 8)) & 0x100f00f00f00f00full;
+  x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+  x = (x | (x << 2)) & 0x1249249249249249ull;
+  return x;
+}
+
+__device__ __forceinline__ uint64_t spread2_31(uint32_t a) {
+  uint64_t x = a & 0x7fffffffull;
+  x = (x | (x << 16)) & 0x0000ffff0000ffffull;
+  x = (x | (x << 8)) & 0x00ff00ff00ff00ffull;
+  x = (x | (x << 4)) & 0x0f0f0f0f0f0f0f0full;
+  x = (x | (x << 2)) & 0x3333333333333333ull;
+  x = (x | (x << 1)) & 0x5555555555555555ull;
+  return x;
+}
+
+template <typename D, int DIM>
+__device__ __forceinline__ uint64_t morton_code(const D* c, const BuildInfo<D>& in) {
+  constexpr int bits = MortonBits<DIM>::value;
+  const D maxq = (D)((1u << bits) - 1u);
+  uint32_t u[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    D x = (c[d] - in.lo[d]) * in.scale[d];
+    x = x > D(0) ? x : D(0);
+    x = x < maxq ? x : maxq;
+    u[d] = (uint32_t)x;
+  }
+  if constexpr (DIM == 3) {
+    return spread3_21(u[0]) | (spread3_21(u[1]) << 1) | (spread3_21(u[2]) << 2);
+  } else if constexpr (DIM == 2) {
+    return spread2_31(u[0]) | (spread2_31(u[1]) << 1);
+  } else {
+    uint64_t code = 0;
+    for (int b = 0; b < bits; ++b)
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) code |= (uint64_t)((u[d] >> b) & 1u) << (b * DIM + d);
+    return code;
+  }
+}
+
+template <typename T, typename D, int DIM>
+__global__ __launch_bounds__(kBlock) void morton_kernel(const T* __restrict__ pts, int64_t n,
+                                                        const uint32_t* __restrict__ blk_off,
+                                                        const BuildInfo<D>* __restrict__ info_p,
+                                                        uint64_t* __restrict__ keys,
+                                                        uint32_t* __restrict__ vals) {
+  const BuildInfo<D> info = *info_p;
+  const int64_t first = (int64_t)blockIdx.x * kTile + (int64_t)threadIdx.x * kPer;
+  uint32_t flags = 0, cnt = 0;
+  for (int e = 0; e < kPer; ++e) {
+    const int64_t i = first + e;
+    if (i >= n) break;
+    bool ok = true;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) ok = ok && elem_finite(pts[i * DIM + d]);
+    if (ok) { flags |= 1u << e; ++cnt; }
+  }
+  // block exclusive scan of per-thread counts
+  __shared__ uint32_t s_wsum[kBlock / 64];
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  uint32_t incl = cnt;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(incl, o);
+    if (lane >= o) incl += y;
+  }
+  if (lane == 63) s_wsum[w] = incl;
+  __syncthreads();
+  uint32_t off = 0;
+#pragma unroll
+  for (int i = 0; i < kBlock / 64; ++i)
+    if (i < w) off += s_wsum[i];
+  uint32_t pos = blk_off[blockIdx.x] + off + incl - cnt;
+  for (int e = 0; e < kPer; ++e) {
+    if (!((flags >> e) & 1u)) continue;
+    const int64_t i = first + e;
+    D c[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) c[d] = (D)pts[i * DIM + d];
+    keys[pos] = morton_code<D, DIM>(c, info);
+    vals[pos] = (uint32_t)i;
+    ++pos;
+  }
+}
+
+template <typename T, typename D, int DIM>
+__global__ __launch_bounds__(kBlock) void gather_kernel(const T* __restrict__ pts,
+                                                        const uint32_t* __restrict__ order,
+                                                        int64_t nv, PointRec<D, DIM>* __restrict__ rec,
+                                                        int32_t* __restrict__ orig,
+                                                        int32_t* __restrict__ fwd) {
+  const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (p >= nv) return;
+  const uint32_t i = order[p];
+  PointRec<D, DIM> r;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) r.c[d] = (D)pts[(int64_t)i * DIM + d];
+  r.orig = (int32_t)i;
+  rec[p] = r;
+  orig[p] = (int32_t)i;
+  fwd[i] = (int32_t)p;
+}
+
+template <typename D, int DIM>
+__global__ __launch_bounds__(kBlock) void leaf_box_kernel(const PointRec<D, DIM>* __restrict__ rec,
+                                                          int64_t nv, int leaf, int64_t nleaves,
+                                                          int64_t L, BoxRec<D, DIM>* __restrict__ box) {
+  const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (j >= L) return;
+  BoxRec<D, DIM> b;
+#pragma unroll
+  for (int d = 0; d < PadDim<DIM>::value; ++d) {
+    b.lo[d] = d < DIM ? pos_inf<D>() : D(0);
+    b.hi[d] = d < DIM ? -pos_inf<D>() : D(0);
+  }
+  if (j < nleaves) {
+    const int64_t s = j * leaf;
+    const int64_t e = s + leaf < nv ? s + leaf : nv;
+    for (int64_t p = s; p < e; ++p) {
+      const PointRec<D, DIM> r = rec[p];
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        b.lo[d] = r.c[d] < b.lo[d] ? r.c[d] : b.lo[d];
+        b.hi[d] = r.c[d] > b.hi[d] ? r.c[d] : b.hi[d];
+      }
+    }
+  }
+  box[L - 1 + j] = b;
+}
+
+template <typename D, int DIM>
+__global__ __launch_bounds__(kBlock) void level_box_kernel(BoxRec<D, DIM>* __restrict__ box,
+                                                           int64_t first, int64_t count) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= count) return;
+  const int64_t i = first + t;
+  const BoxRec<D, DIM> a = box[2 * i + 1];
+  const BoxRec<D, DIM> c = box[2 * i + 2];
+  BoxRec<D, DIM> b;
+#pragma unroll
+  for (int d = 0; d < PadDim<DIM>::value; ++d) {
+    b.lo[d] = a.lo[d] < c.lo[d] ? a.lo[d] : c.lo[d];
+    b.hi[d] = a.hi[d] > c.hi[d] ? a.hi[d] : c.hi[d];
+  }
+  box[i] = b;
+}
+
+unsigned grid_for(int64_t n, int block) {
+  const int64_t g = ceil_div(n, block);
+  return (unsigned)(g > 0 ? g : 1);
+}
+
+template <typename T, typename D, int DIM>
+void build_impl(pptk_tree* t, const T* dpts, int64_t n, hipStream_t s) {
+  // K0: validity + bounding box
+  const int64_t nblk = n > 0 ? ceil_div(n, kTile) : 0;
+  DevBuf blk_cnt((size_t)std::max<int64_t>(nblk, 1) * 4);
+  DevBuf blk_off((size_t)std::max<int64_t>(nblk, 1) * 4);
+  DevBuf blk_lo((size_t)std::max<int64_t>(nblk, 1) * DIM * sizeof(D));
+  DevBuf blk_hi((size_t)std::max<int64_t>(nblk, 1) * DIM * sizeof(D));
+  DevBuf info_d(sizeof(BuildInfo<D>));
+  if (nblk > 0) {
+    ingest_reduce_kernel<T, D, DIM><<<(unsigned)nblk, kBlock, 0, s>>>(
+        dpts, n, blk_cnt.as<uint32_t>(), blk_lo.as<D>(), blk_hi.as<D>());
+    PK_LAUNCH_CHECK();
+  }
+  ingest_finalize_kernel<D, DIM><<<1, kBlock, 0, s>>>(blk_cnt.as<uint32_t>(), blk_lo.as<D>(),
+                                                      blk_hi.as<D>(), nblk,
+                                                      blk_off.as<uint32_t>(),
+                                                      info_d.as<BuildInfo<D>>());
+  PK_LAUNCH_CHECK();
+  BuildInfo<D> info;
+  PK_HIP(hipMemcpyAsync(&info, info_d.get(), sizeof(info), hipMemcpyDeviceToHost, s));
+  PK_HIP(hipStreamSynchronize(s));
+  const int64_t nv = info.n_valid;
+  t->n_valid = nv;
+  for (int d = 0; d < 4; ++d) {
+    t->lo[d] = d < DIM ? (double)info.lo[d] : 0.0;
+    t->hi[d] = d < DIM ? (double)info.hi[d] : 0.0;
+  }
+
+  // Bucket geometry.
+  const int64_t B = t->leaf;
+  t->nleaves = ceil_div(nv, B);
+  int64_t L = 1;
+  int depth = 0;
+  while (L < t->nleaves) { L <<= 1; ++depth; }
+  t->L = L;
+  t->depth = depth;
+
+  t->pts.reset((size_t)std::max<int64_t>(nv, 1) * sizeof(PointRec<D, DIM>));
+  t->orig.reset((size_t)std::max<int64_t>(nv, 1) * 4);
+  t->fwd.reset((size_t)std::max<int64_t>(n, 1) * 4);
+  t->box.reset((size_t)(2 * L - 1) * sizeof(BoxRec<D, DIM>));
+  if (n > 0) PK_HIP(hipMemsetAsync(t->fwd.get(), 0xff, (size_t)n * 4, s));
+
+  if (nv > 0) {
+    // K1: compaction + Morton codes
+    DevBuf keys((size_t)nv * 8);
+    DevBuf vals((size_t)nv * 4);
+    morton_kernel<T, D, DIM><<<(unsigned)nblk, kBlock, 0, s>>>(
+        dpts, n, blk_off.as<uint32_t>(), info_d.as<BuildInfo<D>>(), keys.as<uint64_t>(),
+        vals.as<uint32_t>());
+    PK_LAUNCH_CHECK();
+    // K2: sort
+    radix_sort_pairs(keys.as<uint64_t>(), vals.as<uint32_t>(), nv, DIM * MortonBits<DIM>::value, s);
+    // K3: gather in tree order
+    gather_kernel<T, D, DIM><<<grid_for(nv, kBlock), kBlock, 0, s>>>(
+        dpts, vals.as<uint32_t>(), nv, t->pts.as<PointRec<D, DIM>>(), t->orig.as<int32_t>(),
+        t->fwd.as<int32_t>());
+    PK_LAUNCH_CHECK();
+    PK_HIP(hipStreamSynchronize(s));  // keys/vals return to the pool
+  }
+  leaf_box_kernel<D, DIM><<<grid_for(L, kBlock), kBlock, 0, s>>>(
+      t->pts.as<PointRec<D, DIM>>(), nv, (int)B, t->nleaves, L, t->box.as<BoxRec<D, DIM>>());
+  PK_LAUNCH_CHECK();
+  for (int lvl = depth - 1; lvl >= 0; --lvl) {
+    const int64_t first = ((int64_t)1 << lvl) - 1;
+    const int64_t count = (int64_t)1 << lvl;
+    level_box_kernel<D, DIM><<<grid_for(count, kBlock), kBlock, 0, s>>>(
+        t->box.as<BoxRec<D, DIM>>(), first, count);
+    PK_LAUNCH_CHECK();
+  }
+  PK_HIP(hipStreamSynchronize(s));
+}
+
+std::mutex g_trees_mu;
+std::set<const void*> g_trees;
+
+}  // namespace
+
+int default_leaf_size(int dim, bool dbl, int64_t n) {
+  (void)dim; (void)dbl; (void)n;
+  return 16;
+}
+
+const pptk_tree* check_tree(const pptk_tree* t) {
+  if (!t) fail(PPTK_E_TYPE, "null tree");
+  std::lock_guard<std::mutex> lk(g_trees_mu);
+  if (!g_trees.count(t) || t->magic != pptk_tree::kMagic)
+    fail(PPTK_E_TYPE, "object is not a live k-d tree handle");
+  return t;
+}
+
+}  // namespace pk
+
+namespace pk {
+pptk_status api_tree_build(const void* pts, int64_t n, int dim, pptk_dtype dtype,
+                                       const pptk_build_params* params, pptk_mem mem,
+                                       void* stream, pptk_tree** out) {
+  return pk::guard([&] {
+    if (!out) pk::fail(PPTK_E_VALUE, "null output handle");
+    *out = nullptr;
+    if (dim < 2 || dim > 4) pk::fail(PPTK_E_RUNTIME, "Invalid k-d tree dim = %d.", dim);
+    if (!pk::dtype_valid(dtype)) pk::fail(PPTK_E_RUNTIME, "Invalid k-d tree type_num = %d.", (int)dtype);
+    if (n < 0) pk::fail(PPTK_E_VALUE, "number of points must be non-negative");
+    if (n > 0x7fffffffLL) pk::fail(PPTK_E_VALUE, "at most 2^31-1 points per tree (int32 indices)");
+    if (n > 0 && !pts) pk::fail(PPTK_E_VALUE, "null points");
+    hipStream_t s = pk::pick_stream(stream);
+    std::unique_ptr<pptk_tree> t(new pptk_tree());
+    t->dim = dim;
+    t->dtype = dtype;
+    t->dist = dtype == PPTK_F64 ? PPTK_F64 : PPTK_F32;
+    t->n_input = n;
+    int leaf = params && params->max_leaf_size > 0 ? params->max_leaf_size
+                                                   : pk::default_leaf_size(dim, t->dist == PPTK_F64, n);
+    t->leaf = std::min(64, std::max(1, leaf));
+    pk::DevInput in;
+    pk::stage_input(in, pts, (size_t)n * dim * pk::dtype_size(dtype), mem, s);
+    pptk_tree* tp = t.get();
+    const void* p = in.ptr;
+    auto run = [&](auto tag) {
+        using T = typename decltype(tag)::type;
+        pptk_tree* t = tp;
+        if (t->dist == PPTK_F64) {
+          switch (t->dim) {
+            case 2: pk::build_impl<T, double, 2>(t, (const T*)p, n, s); break;
+            case 3: pk::build_impl<T, double, 3>(t, (const T*)p, n, s); break;
+            default: pk::build_impl<T, double, 4>(t, (const T*)p, n, s); break;
+          }
+        } else {
+          switch (t->dim) {
+            case 2: pk::build_impl<T, float, 2>(t, (const T*)p, n, s); break;
+            case 3: pk::build_impl<T, float, 3>(t, (const T*)p, n, s); break;
+            default: pk::build_impl<T, float, 4>(t, (const T*)p, n, s); break;
+          }
+        }
+    };
+    pk::dispatch_dtype(dtype, run);
+    {
+      std::lock_guard<std::mutex> lk(pk::g_trees_mu);
+      pk::g_trees.insert(t.get());
+    }
+    *out = t.release();
+  });
+}
+}  // namespace pk
+
+extern "C" pptk_status pptk_tree_build(const void* pts, int64_t n, int dim, pptk_dtype dtype,
+                                       const pptk_build_params* params, pptk_mem mem,
+                                       void* stream, pptk_tree** out) {
+  return pk::api_tree_build(pts, n, dim, dtype, params, mem, stream, out);
+}
+
+extern "C" void pptk_tree_free(pptk_tree* tree) {
+  if (!tree) return;
+  {
+    std::lock_guard<std::mutex> lk(pk::g_trees_mu);
+    if (!pk::g_trees.erase(tree)) return;
+  }
+  tree->magic = 0;
+  delete tree;
+}
+
+extern "C" int pptk_tree_check(const void* p) {
+  std::lock_guard<std::mutex> lk(pk::g_trees_mu);
+  return p && pk::g_trees.count(p) ? 1 : 0;
+}
+
+extern "C" pptk_status pptk_tree_get_info(const pptk_tree* tree, pptk_tree_info* out) {
+  return pk::guard([&] {
+    pk::check_tree(tree);
+    if (!out) pk::fail(PPTK_E_VALUE, "null output");
+    out->dim = tree->dim;
+    out->dtype = tree->dtype;
+    out->dist_dtype = tree->dist;
+    out->num_input = tree->n_input;
+    out->num_valid = tree->n_valid;
+    out->leaf_size = tree->leaf;
+    out->num_leaves = tree->nleaves;
+    out->leaf_slots = tree->L;
+    out->depth = tree->depth;
+    for (int d = 0; d < 4; ++d) { out->bbox_lo[d] = tree->lo[d]; out->bbox_hi[d] = tree->hi[d]; }
+    out->device_bytes = (int64_t)(tree->pts.bytes() + tree->orig.bytes() + tree->fwd.bytes() +
+                                  tree->box.bytes());
+  });
+}
+
+namespace pk {
+namespace {
+template <typename D, int DIM>
+__global__ __launch_bounds__(256) void export_points_kernel(const PointRec<D, DIM>* __restrict__ rec,
+                                                            int64_t nv, D* __restrict__ out) {
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= nv) return;
+  const PointRec<D, DIM> r = rec[p];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) out[p * DIM + d] = r.c[d];
+}
+template <typename D, int DIM>
+__global__ __launch_bounds__(256) void export_boxes_kernel(const BoxRec<D, DIM>* __restrict__ box,
+                                                           int64_t nn, D* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= nn) return;
+  const BoxRec<D, DIM> b = box[i];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    out[i * 2 * DIM + d] = b.lo[d];
+    out[i * 2 * DIM + DIM + d] = b.hi[d];
+  }
+}
+}  // namespace
+}  // namespace pk
+
+namespace pk {
+pptk_status api_tree_export(const pptk_tree* tree, void* points, int32_t* reverse,
+                                        int32_t* forward, void* boxes, pptk_mem mem) {
+  return pk::guard([&] {
+    pk::check_tree(tree);
+    hipStream_t s = pk::lib_stream();
+    const int64_t nv = tree->n_valid;
+    const int64_t nn = 2 * tree->L - 1;
+    const pptk_tree* t = tree;
+    auto run = [&](auto dt, auto dimc) {
+        using D = typename decltype(dt)::type;
+        constexpr int DIM = decltype(dimc)::value;
+        if (points && nv > 0) {
+          pk::DevOutput o;
+          o.init(points, (size_t)nv * DIM * sizeof(D), mem);
+          pk::export_points_kernel<D, DIM><<<(unsigned)pk::ceil_div(nv, 256), 256, 0, s>>>(
+              t->pts.as<const pk::PointRec<D, DIM>>(), nv, (D*)o.dptr);
+          PK_LAUNCH_CHECK();
+          o.finish(s);
+          PK_HIP(hipStreamSynchronize(s));
+        }
+        if (boxes) {
+          pk::DevOutput o;
+          o.init(boxes, (size_t)nn * 2 * DIM * sizeof(D), mem);
+          pk::export_boxes_kernel<D, DIM><<<(unsigned)pk::ceil_div(nn, 256), 256, 0, s>>>(
+              t->box.as<const pk::BoxRec<D, DIM>>(), nn, (D*)o.dptr);
+          PK_LAUNCH_CHECK();
+          o.finish(s);
+          PK_HIP(hipStreamSynchronize(s));
+        }
+    };
+    pk::dispatch_tree(tree, run);
+    const hipMemcpyKind k = mem == PPTK_MEM_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+    if (reverse && nv > 0) PK_HIP(hipMemcpyAsync(reverse, tree->orig.get(), (size_t)nv * 4, k, s));
+    if (forward && tree->n_input > 0)
+      PK_HIP(hipMemcpyAsync(forward, tree->fwd.get(), (size_t)tree->n_input * 4, k, s));
+    PK_HIP(hipStreamSynchronize(s));
+  });
+}
+}  // namespace pk
+
+extern "C" pptk_status pptk_tree_export(const pptk_tree* tree, void* points, int32_t* reverse,
+                                        int32_t* forward, void* boxes, pptk_mem mem) {
+  return pk::api_tree_export(tree, points, reverse, forward, boxes, mem);
+}

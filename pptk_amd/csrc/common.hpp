@@ -1,0 +1,239 @@
+// Internal runtime shared by the library's translation units: error
+// reporting, the device allocator, the library stream, element-type helpers
+// and the device-side geometry primitives (distance, box bounds).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <string>
+#include <type_traits>
+
+#include "../../include/pptk_amd.h"
+
+namespace pk {
+
+// ---------------------------------------------------------------- errors ----
+
+struct Error {
+  pptk_status status;
+  std::string msg;
+};
+
+[[noreturn]] void fail(pptk_status st, const char* fmt, ...);
+void set_last_error(const std::string& msg);
+void clear_last_error();
+
+#define PK_HIP(expr)                                                          \
+  do {                                                                        \
+    hipError_t e_ = (expr);                                                   \
+    if (e_ != hipSuccess)                                                     \
+      ::pk::fail(PPTK_E_DEVICE, "%s failed: %s (%s:%d)", #expr,               \
+                 hipGetErrorString(e_), __FILE__, __LINE__);                  \
+  } while (0)
+
+#define PK_LAUNCH_CHECK() PK_HIP(hipGetLastError())
+
+// Runs `body` and converts pk::Error / std::bad_alloc into a status code.
+template <typename F>
+pptk_status guard(F&& body) {
+  try {
+    clear_last_error();
+    body();
+    return PPTK_OK;
+  } catch (const Error& e) {
+    set_last_error(e.msg);
+    return e.status;
+  } catch (const std::bad_alloc&) {
+    set_last_error("host allocation failed");
+    return PPTK_E_MEMORY;
+  } catch (const std::exception& e) {
+    set_last_error(e.what());
+    return PPTK_E_RUNTIME;
+  }
+}
+
+// ---------------------------------------------------------------- device ----
+
+// Selects the device, creates the library stream on first use.
+void ensure_device();
+hipStream_t lib_stream();
+inline hipStream_t pick_stream(void* s) {
+  return s ? reinterpret_cast<hipStream_t>(s) : lib_stream();
+}
+int device_cu_count();
+
+// Pooled device allocation. Blocks are cached after free and reused for
+// requests of up to 1.25x smaller size; pptk_trim() returns them.
+void* dev_alloc(size_t bytes);
+void dev_free(void* p);
+
+// RAII device buffer backed by the pool.
+class DevBuf {
+ public:
+  DevBuf() = default;
+  explicit DevBuf(size_t bytes) { reset(bytes); }
+  ~DevBuf() { release(); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) { release(); p_ = o.p_; n_ = o.n_; o.p_ = nullptr; o.n_ = 0; }
+    return *this;
+  }
+  void reset(size_t bytes) {
+    release();
+    if (bytes) { p_ = dev_alloc(bytes); n_ = bytes; }
+  }
+  void release() {
+    if (p_) dev_free(p_);
+    p_ = nullptr; n_ = 0;
+  }
+  template <typename T> T* as() const { return reinterpret_cast<T*>(p_); }
+  void* get() const { return p_; }
+  size_t bytes() const { return n_; }
+
+ private:
+  void* p_ = nullptr;
+  size_t n_ = 0;
+};
+
+// Copies `bytes` from a host or device source into a fresh device buffer,
+// or returns the device pointer itself when the source is already there.
+struct DevInput {
+  DevBuf owned;
+  const void* ptr = nullptr;
+};
+void stage_input(DevInput& out, const void* src, size_t bytes, pptk_mem mem, hipStream_t s);
+
+// Output buffer: either the caller's device pointer, or a device temporary
+// copied back to the caller's host pointer by `finish`.
+struct DevOutput {
+  DevBuf owned;
+  void* dptr = nullptr;
+  void* host = nullptr;
+  size_t bytes = 0;
+  void init(void* user, size_t nbytes, pptk_mem mem);
+  void finish(hipStream_t s);
+};
+
+// ------------------------------------------------------------ dtype util ----
+
+size_t dtype_size(pptk_dtype t);
+bool dtype_valid(int t);
+inline bool dist_is_double(pptk_dtype t) { return t == PPTK_F64; }
+
+template <typename T> struct Tag { using type = T; };
+
+// Calls f(Tag<T>{}) with T the C++ type of `t`.
+template <typename F>
+void dispatch_dtype(pptk_dtype t, F&& f) {
+  switch (t) {
+    case PPTK_F32: f(Tag<float>{}); break;
+    case PPTK_F64: f(Tag<double>{}); break;
+    case PPTK_I8: f(Tag<int8_t>{}); break;
+    case PPTK_I16: f(Tag<int16_t>{}); break;
+    case PPTK_I32: f(Tag<int32_t>{}); break;
+    case PPTK_I64: f(Tag<int64_t>{}); break;
+    case PPTK_U8: f(Tag<uint8_t>{}); break;
+    case PPTK_U16: f(Tag<uint16_t>{}); break;
+    case PPTK_U32: f(Tag<uint32_t>{}); break;
+    case PPTK_U64: f(Tag<uint64_t>{}); break;
+    default: fail(PPTK_E_TYPE, "unsupported dtype code %d", (int)t);
+  }
+}
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// -------------------------------------------------------- device helpers ----
+
+template <typename D> __host__ __device__ inline D pos_inf();
+template <> __host__ __device__ inline float pos_inf<float>() { return __builtin_huge_valf(); }
+template <> __host__ __device__ inline double pos_inf<double>() { return __builtin_huge_val(); }
+
+// Point record in tree order: coordinates in the distance type plus the
+// original row index, padded to 16-byte multiples so one record is one or
+// two vector loads.
+template <typename D, int DIM>
+struct alignas(16) PointRec {
+  D c[DIM];
+  int32_t orig;
+};
+
+// Node box record: lo[PD], hi[PD] (PD = 4 for 3-D so a box is two float4s).
+template <int DIM> struct PadDim { static constexpr int value = DIM == 3 ? 4 : DIM; };
+template <typename D, int DIM>
+struct alignas(16) BoxRec {
+  D lo[PadDim<DIM>::value];
+  D hi[PadDim<DIM>::value];
+};
+
+// Squared distance exactly as pointkd::PointDistance2 (kdtree-impl.h:331-339):
+// running sum from zero, each term (DistT)b - (DistT)a squared. The library is
+// compiled with -ffp-contract=off, so no term is fused into an FMA.
+template <typename D, int DIM>
+__device__ __forceinline__ D dist2(const D* q, const D* p) {
+  D s = D(0);
+#pragma unroll
+  for (int i = 0; i < DIM; ++i) {
+    D t = p[i] - q[i];
+    s = s + t * t;
+  }
+  return s;
+}
+
+// Lower bound of dist2 over a box, same rounding as pointkd::MinDist2
+// (pptk/kdtree/src/box.h:157-189). Monotone rounding makes it <= dist2(q,p)
+// for every p inside the box, so pruning on it never changes a result set.
+template <typename D, int DIM>
+__device__ __forceinline__ D box_min_dist2(const BoxRec<D, DIM>& b, const D* q) {
+  D s = D(0);
+#pragma unroll
+  for (int i = 0; i < DIM; ++i) {
+    D dmin = b.lo[i] - q[i];
+    D dmax = b.hi[i] - q[i];
+    D v = dmin > D(0) ? dmin * dmin : (dmax < D(0) ? dmax * dmax : D(0));
+    s = s + v;
+  }
+  return s;
+}
+
+// Upper bound of dist2 over a box, same rounding as pointkd::MaxDist2
+// (box.h:171-198): >= dist2(q,p) for every p inside the box.
+template <typename D, int DIM>
+__device__ __forceinline__ D box_max_dist2(const BoxRec<D, DIM>& b, const D* q) {
+  D s = D(0);
+#pragma unroll
+  for (int i = 0; i < DIM; ++i) {
+    D dmin = b.lo[i] - q[i];
+    D dmax = b.hi[i] - q[i];
+    D a = dmin * dmin, c = dmax * dmax;
+    s = s + (a > c ? a : c);
+  }
+  return s;
+}
+
+template <typename D> inline D qnan_host() { return std::numeric_limits<D>::quiet_NaN(); }
+
+template <typename D> __device__ __forceinline__ bool finite_val(D x) {
+  return x - x == D(0);  // false for NaN and +-inf
+}
+
+// Device view of a built tree, passed by value to the query kernels.
+template <typename D, int DIM>
+struct TreeView {
+  const PointRec<D, DIM>* pts;  // num_valid records, tree order
+  const int32_t* orig;          // num_valid, tree position -> original index
+  const int32_t* fwd;           // num_input, original index -> position or -1
+  const BoxRec<D, DIM>* box;    // 2*L-1 nodes, heap order
+  int32_t n;                    // num_valid
+  int32_t leaf;                 // bucket size B
+  int32_t L;                    // leaf slots (power of two)
+  int32_t depth;                // log2(L)
+  int64_t n_input;
+};
+
+}  // namespace pk

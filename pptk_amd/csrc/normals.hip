@@ -1,0 +1,509 @@
+// Fused PCA normals: one lane per output row runs the neighbourhood search on
+// the tree (kNN including the point itself, or r-near for k == -1), forms the
+// covariance of the centred neighbourhood, C = X^T X / n, and solves the 3x3
+// symmetric eigenproblem in registers.
+//
+// Reference: estimate_normals<T> (pptk/processing/estimate_normals/
+// estimate_normals.cpp:64-111): neighbourhood via KNearestNeighbors(q, k,
+// (T)r) or RNearNeighbors(q, (T)r), X -= colwise mean, C = X^T X / n,
+// Eigen::SelfAdjointEigenSolver, eigenvalues ascending.
+//
+// Precision modes:
+//   MIXED : neighbour offsets e = p - q relative to the query point (the
+//           centroid shift), mean and covariance accumulated in FP32 as
+//           double-float pairs (TwoSum for additions, FMA-exact TwoProduct
+//           for products; Ogita, Rump & Oishi, "Accurate sum and dot
+//           product", SIAM J. Sci. Comput. 26(6), 2005), FP32 Jacobi; rows
+//           whose eigengap (l1 - l0) / l2 < 1e-3 (or, with all eigenvectors,
+//           (l2 - l1) / l2 < 1e-3) are re-solved in FP64 from the same
+//           double-float covariance.
+//   FP64  : offsets, covariance and Jacobi in double.
+#include <algorithm>
+
+#include "eigen3.hpp"
+#include "query.hpp"
+#include "traverse.hpp"
+#include "tree.hpp"
+
+namespace pk {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr double kGapThreshold = 1e-3;
+
+__device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
+  s = a + b;
+  const float bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+
+// Double-float running sum.
+struct DF {
+  float hi, lo;
+  __device__ __forceinline__ void zero() { hi = 0.f; lo = 0.f; }
+  __device__ __forceinline__ void add(float x) {
+    float s, e;
+    two_sum(hi, x, s, e);
+    hi = s;
+    lo = lo + e;
+  }
+  __device__ __forceinline__ void add_prod(float x, float y) {
+    const float p = x * y;
+    const float pe = fmaf(x, y, -p);  // exact product error
+    add(p);
+    lo = lo + pe;
+  }
+  __device__ __forceinline__ double value() const { return (double)hi + (double)lo; }
+};
+
+template <typename D>
+__device__ __forceinline__ D qnan() { return (D)__builtin_nan(""); }
+
+template <typename D, int NE>
+__device__ __forceinline__ void write_nan(int64_t row, D* evecs, D* evals, int32_t* sizes) {
+  const D nn = qnan<D>();
+  if (evecs)
+#pragma unroll
+    for (int i = 0; i < 3 * NE; ++i) evecs[row * 3 * NE + i] = nn;
+  if (evals)
+#pragma unroll
+    for (int i = 0; i < NE; ++i) evals[row * NE + i] = nn;
+  if (sizes) sizes[row] = 0;
+}
+
+template <typename D, typename F, int NE>
+__device__ __forceinline__ void write_eig(int64_t row, const F (&w)[3], const F (&v)[3][3],
+                                          int n, D* evecs, D* evals, int32_t* sizes) {
+  if (evecs) {
+#pragma unroll
+    for (int j = 0; j < NE; ++j)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) evecs[row * 3 * NE + j * 3 + c] = (D)v[c][j];
+  }
+  if (evals) {
+#pragma unroll
+    for (int j = 0; j < NE; ++j) evals[row * NE + j] = (D)w[j];
+  }
+  if (sizes) sizes[row] = n;
+}
+
+// Covariance of the neighbourhood enumerated by `each(f)` (f receives tree
+// positions), centred, divided by n. Two passes: mean of the offsets from
+// q, then the centred products.
+template <typename D, int PREC, typename Each>
+__device__ __forceinline__ void knn_covariance(const TreeView<D, 3>& tv, const D* q, int n,
+                                               Each each, double (&cd)[3][3], float (&cf)[3][3]) {
+  if constexpr (PREC == PPTK_PREC_MIXED) {
+    DF s[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) s[a].zero();
+    each([&](int32_t p) {
+      const PointRec<D, 3> r = tv.pts[p];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) s[a].add((float)(r.c[a] - q[a]));
+    });
+    float mu[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) mu[a] = (float)(s[a].value() / (double)n);
+    DF c[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) c[i].zero();
+    each([&](int32_t p) {
+      const PointRec<D, 3> r = tv.pts[p];
+      float e[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) e[a] = (float)(r.c[a] - q[a]) - mu[a];
+      c[0].add_prod(e[0], e[0]);
+      c[1].add_prod(e[0], e[1]);
+      c[2].add_prod(e[0], e[2]);
+      c[3].add_prod(e[1], e[1]);
+      c[4].add_prod(e[1], e[2]);
+      c[5].add_prod(e[2], e[2]);
+    });
+    const double inv = 1.0 / (double)n;
+    const double v[6] = {c[0].value() * inv, c[1].value() * inv, c[2].value() * inv,
+                         c[3].value() * inv, c[4].value() * inv, c[5].value() * inv};
+    cd[0][0] = v[0]; cd[0][1] = v[1]; cd[0][2] = v[2];
+    cd[1][0] = v[1]; cd[1][1] = v[3]; cd[1][2] = v[4];
+    cd[2][0] = v[2]; cd[2][1] = v[4]; cd[2][2] = v[5];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) cf[i][j] = (float)cd[i][j];
+  } else {
+    double s[3] = {0.0, 0.0, 0.0};
+    each([&](int32_t p) {
+      const PointRec<D, 3> r = tv.pts[p];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) s[a] = s[a] + ((double)r.c[a] - (double)q[a]);
+    });
+    double mu[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) mu[a] = s[a] / (double)n;
+    double c[6] = {0, 0, 0, 0, 0, 0};
+    each([&](int32_t p) {
+      const PointRec<D, 3> r = tv.pts[p];
+      double e[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) e[a] = ((double)r.c[a] - (double)q[a]) - mu[a];
+      c[0] = c[0] + e[0] * e[0];
+      c[1] = c[1] + e[0] * e[1];
+      c[2] = c[2] + e[0] * e[2];
+      c[3] = c[3] + e[1] * e[1];
+      c[4] = c[4] + e[1] * e[2];
+      c[5] = c[5] + e[2] * e[2];
+    });
+    const double inv = 1.0 / (double)n;
+    cd[0][0] = c[0] * inv; cd[0][1] = c[1] * inv; cd[0][2] = c[2] * inv;
+    cd[1][0] = c[1] * inv; cd[1][1] = c[3] * inv; cd[1][2] = c[4] * inv;
+    cd[2][0] = c[2] * inv; cd[2][1] = c[4] * inv; cd[2][2] = c[5] * inv;
+    (void)cf;
+  }
+}
+
+// Solves and writes one row. Returns true if the FP64 fallback was taken.
+template <typename D, int PREC, int NE>
+__device__ __forceinline__ bool solve_and_write(int64_t row, int n, double (&cd)[3][3],
+                                                float (&cf)[3][3], D* evecs, D* evals,
+                                                int32_t* sizes) {
+  if constexpr (PREC == PPTK_PREC_MIXED) {
+    float w[3], v[3][3];
+    eig_sym3<float>(cf, w, v);
+    const float l2 = w[2];
+    bool fb = !(l2 > 0.f) || ((double)(w[1] - w[0]) < kGapThreshold * (double)l2);
+    if (NE == 3) fb = fb || ((double)(w[2] - w[1]) < kGapThreshold * (double)l2);
+    if (!fb) {
+      write_eig<D, float, NE>(row, w, v, n, evecs, evals, sizes);
+      return false;
+    }
+  }
+  double w[3], v[3][3];
+  eig_sym3<double>(cd, w, v);
+  write_eig<D, double, NE>(row, w, v, n, evecs, evals, sizes);
+  return PREC == PPTK_PREC_MIXED;
+}
+
+__device__ __forceinline__ void count_fallbacks(bool fb, unsigned long long* nfb) {
+  const uint64_t b = __ballot(fb);
+  const uint64_t act = __ballot(true);
+  if (b && (int)(threadIdx.x & 63) == __builtin_ctzll(act))
+    atomicAdd(nfb, (unsigned long long)__popcll(b));
+}
+
+// Resolves row `t`: tree-order mode (rows == nullptr, t = tree position) or
+// explicit rows (t = output row, rows[t] = original index).
+template <typename D>
+__device__ __forceinline__ void resolve_row(const TreeView<D, 3>& tv, const int32_t* rows, int64_t t,
+                                            D* q, int64_t& row, bool& ok) {
+  ok = true;
+  if (!rows) {
+    const PointRec<D, 3> r = tv.pts[t];
+    row = r.orig;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) q[d] = r.c[d];
+  } else {
+    row = t;
+    const int32_t o = rows[t];
+    const int32_t p = (o >= 0 && (int64_t)o < tv.n_input) ? tv.fwd[o] : -1;
+    if (p < 0) {
+      ok = false;
+    } else {
+      const PointRec<D, 3> r = tv.pts[p];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) q[d] = r.c[d];
+    }
+  }
+}
+
+template <typename D, int K, int PREC, int NE>
+__global__ __launch_bounds__(kBlock) void normals_knn_kernel(TreeView<D, 3> tv,
+                                                             const int32_t* __restrict__ rows,
+                                                             int64_t m, int k, D r2,
+                                                             D* __restrict__ evecs,
+                                                             D* __restrict__ evals,
+                                                             int32_t* __restrict__ sizes,
+                                                             unsigned long long* __restrict__ nfb) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= m) return;
+  D q[3];
+  int64_t row;
+  bool ok;
+  resolve_row<D>(tv, rows, t, q, row, ok);
+  TopK<D, K, true> st;
+  st.init(k);
+  if (ok) knn_search<D, 3>(tv, q, r2, -1, st);
+  int n = 0;
+#pragma unroll
+  for (int s = 0; s < K; ++s) n += (s >= K - k && st.d[s] < pos_inf<D>()) ? 1 : 0;
+  bool fb = false;
+  if (n == 0) {
+    write_nan<D, NE>(row, evecs, evals, sizes);
+  } else {
+    auto each = [&](auto f) {
+#pragma unroll
+      for (int s = 0; s < K; ++s)
+        if (s >= K - k && st.d[s] < pos_inf<D>()) f(st.p[s]);
+    };
+    double cd[3][3];
+    float cf[3][3];
+    knn_covariance<D, PREC>(tv, q, n, each, cd, cf);
+    fb = solve_and_write<D, PREC, NE>(row, n, cd, cf, evecs, evals, sizes);
+  }
+  count_fallbacks(fb, nfb);
+}
+
+template <typename D, int PREC, int NE>
+__global__ __launch_bounds__(kBlock) void normals_heap_kernel(TreeView<D, 3> tv,
+                                                              const int32_t* __restrict__ rows,
+                                                              int64_t t0, int64_t t1, int k, D r2,
+                                                              D* __restrict__ ws_d,
+                                                              int32_t* __restrict__ ws_o,
+                                                              int32_t* __restrict__ ws_p,
+                                                              D* __restrict__ evecs,
+                                                              D* __restrict__ evals,
+                                                              int32_t* __restrict__ sizes,
+                                                              unsigned long long* __restrict__ nfb) {
+  const int64_t t = t0 + (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= t1) return;
+  D q[3];
+  int64_t row;
+  bool ok;
+  resolve_row<D>(tv, rows, t, q, row, ok);
+  const int64_t w0 = (t - t0) * k;
+  HeapK<D> st;
+  st.init(ws_d + w0, ws_o + w0, ws_p + w0, k);
+  if (ok) knn_search<D, 3>(tv, q, r2, -1, st);
+  const int n = st.size;
+  bool fb = false;
+  if (n == 0) {
+    write_nan<D, NE>(row, evecs, evals, sizes);
+  } else {
+    const int32_t* hp = ws_p + w0;
+    auto each = [&](auto f) {
+      for (int s = 0; s < n; ++s) f(hp[s]);
+    };
+    double cd[3][3];
+    float cf[3][3];
+    knn_covariance<D, PREC>(tv, q, n, each, cd, cf);
+    fb = solve_and_write<D, PREC, NE>(row, n, cd, cf, evecs, evals, sizes);
+  }
+  count_fallbacks(fb, nfb);
+}
+
+// k == -1: neighbourhood = every point with d < r*r (RNearNeighbors,
+// estimate_normals.cpp:79-80). One pass of shifted sums around q, combined
+// as C = S2/n - mu mu^T.
+template <typename D, int PREC, int NE>
+__global__ __launch_bounds__(kBlock) void normals_rnear_kernel(TreeView<D, 3> tv,
+                                                               const int32_t* __restrict__ rows,
+                                                               int64_t m, D r2,
+                                                               D* __restrict__ evecs,
+                                                               D* __restrict__ evals,
+                                                               int32_t* __restrict__ sizes,
+                                                               unsigned long long* __restrict__ nfb) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= m) return;
+  D q[3];
+  int64_t row;
+  bool ok;
+  resolve_row<D>(tv, rows, t, q, row, ok);
+  int64_t n = 0;
+  double s1[3] = {0, 0, 0};
+  double s2[6] = {0, 0, 0, 0, 0, 0};
+  DF f1[3], f2[6];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) f1[i].zero();
+#pragma unroll
+  for (int i = 0; i < 6; ++i) f2[i].zero();
+  if (ok) {
+    auto visit = [&](const PointRec<D, 3>& r) {
+      ++n;
+      if constexpr (PREC == PPTK_PREC_MIXED) {
+        float e[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) { e[a] = (float)(r.c[a] - q[a]); f1[a].add(e[a]); }
+        f2[0].add_prod(e[0], e[0]); f2[1].add_prod(e[0], e[1]); f2[2].add_prod(e[0], e[2]);
+        f2[3].add_prod(e[1], e[1]); f2[4].add_prod(e[1], e[2]); f2[5].add_prod(e[2], e[2]);
+      } else {
+        double e[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) { e[a] = (double)r.c[a] - (double)q[a]; s1[a] = s1[a] + e[a]; }
+        s2[0] = s2[0] + e[0] * e[0]; s2[1] = s2[1] + e[0] * e[1]; s2[2] = s2[2] + e[0] * e[2];
+        s2[3] = s2[3] + e[1] * e[1]; s2[4] = s2[4] + e[1] * e[2]; s2[5] = s2[5] + e[2] * e[2];
+      }
+    };
+    rnear_visit<D, 3>(tv, q, r2, visit);
+  }
+  bool fb = false;
+  if (n == 0) {
+    write_nan<D, NE>(row, evecs, evals, sizes);
+  } else {
+    if constexpr (PREC == PPTK_PREC_MIXED) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) s1[i] = f1[i].value();
+#pragma unroll
+      for (int i = 0; i < 6; ++i) s2[i] = f2[i].value();
+    }
+    const double inv = 1.0 / (double)n;
+    const double mu[3] = {s1[0] * inv, s1[1] * inv, s1[2] * inv};
+    double cd[3][3];
+    cd[0][0] = s2[0] * inv - mu[0] * mu[0];
+    cd[0][1] = s2[1] * inv - mu[0] * mu[1];
+    cd[0][2] = s2[2] * inv - mu[0] * mu[2];
+    cd[1][1] = s2[3] * inv - mu[1] * mu[1];
+    cd[1][2] = s2[4] * inv - mu[1] * mu[2];
+    cd[2][2] = s2[5] * inv - mu[2] * mu[2];
+    cd[1][0] = cd[0][1]; cd[2][0] = cd[0][2]; cd[2][1] = cd[1][2];
+    float cf[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) cf[i][j] = (float)cd[i][j];
+    fb = solve_and_write<D, PREC, NE>(row, (int)n, cd, cf, evecs, evals, sizes);
+  }
+  count_fallbacks(fb, nfb);
+}
+
+template <typename D, int NE>
+__global__ __launch_bounds__(kBlock) void nan_rows_kernel(const int32_t* __restrict__ fwd,
+                                                          int64_t nrows, D* __restrict__ evecs,
+                                                          D* __restrict__ evals,
+                                                          int32_t* __restrict__ sizes) {
+  const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (r >= nrows) return;
+  if (fwd[r] < 0) write_nan<D, NE>(r, evecs, evals, sizes);
+}
+
+inline unsigned grid_of(int64_t n) { return (unsigned)std::max<int64_t>(1, ceil_div(n, kBlock)); }
+
+template <typename D, int PREC, int NE>
+void normals_launch(const pptk_tree* t, int k, D r2, const int32_t* drows, int64_t m, D* evecs,
+                    D* evals, int32_t* sizes, unsigned long long* nfb, hipStream_t s) {
+  const TreeView<D, 3> tv = tree_view<D, 3>(t);
+  if (m == 0) return;
+  if (k == -1) {
+    normals_rnear_kernel<D, PREC, NE><<<grid_of(m), kBlock, 0, s>>>(tv, drows, m, r2, evecs,
+                                                                     evals, sizes, nfb);
+    PK_LAUNCH_CHECK();
+  } else if (k <= 32) {
+    auto launch = [&](auto kc) {
+      constexpr int K = decltype(kc)::value;
+      normals_knn_kernel<D, K, PREC, NE><<<grid_of(m), kBlock, 0, s>>>(tv, drows, m, k, r2, evecs,
+                                                                        evals, sizes, nfb);
+      PK_LAUNCH_CHECK();
+    };
+    if (k <= 4) launch(std::integral_constant<int, 4>());
+    else if (k <= 8) launch(std::integral_constant<int, 8>());
+    else if (k <= 16) launch(std::integral_constant<int, 16>());
+    else launch(std::integral_constant<int, 32>());
+  } else {
+    const int64_t per = (int64_t)k * (sizeof(D) + 8);
+    const int64_t chunk = std::max<int64_t>(kBlock, ((int64_t)256 << 20) / per);
+    const int64_t c = std::min(chunk, m);
+    DevBuf wd((size_t)c * k * sizeof(D)), wo((size_t)c * k * 4), wp((size_t)c * k * 4);
+    for (int64_t t0 = 0; t0 < m; t0 += chunk) {
+      const int64_t t1 = std::min(m, t0 + chunk);
+      normals_heap_kernel<D, PREC, NE><<<grid_of(t1 - t0), kBlock, 0, s>>>(
+          tv, drows, t0, t1, k, r2, wd.as<D>(), wo.as<int32_t>(), wp.as<int32_t>(), evecs, evals,
+          sizes, nfb);
+      PK_LAUNCH_CHECK();
+    }
+    PK_HIP(hipStreamSynchronize(s));
+  }
+}
+
+template <typename D>
+void normals_run(const pptk_tree* t, int k, double r, const int32_t* rows, int64_t m_rows,
+                 int num_eigen, pptk_precision prec, void* out_evecs, void* out_evals,
+                 int32_t* out_sizes, int64_t* out_num_fp64, pptk_mem mem, hipStream_t s) {
+  const bool tree_order = rows == nullptr;
+  const int64_t nrows = tree_order ? t->n_input : m_rows;
+  if (!tree_order && mem == PPTK_MEM_HOST) {
+    for (int64_t i = 0; i < m_rows; ++i)
+      if (rows[i] < 0 || rows[i] >= t->n_input)
+        fail(PPTK_E_RUNTIME, "%lld-th subsample index %d is outside of [0,%lld]", (long long)i,
+             (int)rows[i], (long long)t->n_input - 1);
+  }
+  DevInput drow;
+  if (!tree_order) stage_input(drow, rows, (size_t)m_rows * 4, mem, s);
+  DevOutput oe, ov, os;
+  oe.init(out_evecs, (size_t)nrows * 3 * num_eigen * sizeof(D), mem);
+  ov.init(out_evals, (size_t)nrows * num_eigen * sizeof(D), mem);
+  os.init(out_sizes, (size_t)nrows * 4, mem);
+  DevBuf nfb(8);
+  PK_HIP(hipMemsetAsync(nfb.get(), 0, 8, s));
+  D* evecs = (D*)oe.dptr;
+  D* evals = (D*)ov.dptr;
+  int32_t* sizes = (int32_t*)os.dptr;
+  const D rr = (D)r;
+  const D r2 = rr * rr;
+  const int64_t m = tree_order ? t->n_valid : m_rows;
+  const bool all_nan = k < -1;  // KNearestNeighbors with k < 1 returns nothing
+  if (nrows > 0 && (all_nan || (tree_order && t->n_valid < t->n_input))) {
+    // Rows without a valid point (or every row, for k < -1) are NaN / size 0.
+    const int32_t* fwd = t->fwd.as<const int32_t>();
+    if (all_nan) {
+      if (evecs) fill_async<D>(evecs, nrows * 3 * num_eigen, qnan_host<D>(), s);
+      if (evals) fill_async<D>(evals, nrows * num_eigen, qnan_host<D>(), s);
+      if (sizes) fill_async<int32_t>(sizes, nrows, 0, s);
+    } else if (num_eigen == 1) {
+      nan_rows_kernel<D, 1><<<grid_of(nrows), kBlock, 0, s>>>(fwd, nrows, evecs, evals, sizes);
+      PK_LAUNCH_CHECK();
+    } else {
+      nan_rows_kernel<D, 3><<<grid_of(nrows), kBlock, 0, s>>>(fwd, nrows, evecs, evals, sizes);
+      PK_LAUNCH_CHECK();
+    }
+  }
+  if (!all_nan) {
+    const int32_t* dr = tree_order ? nullptr : (const int32_t*)drow.ptr;
+    unsigned long long* c = nfb.as<unsigned long long>();
+    const bool mixed = prec == PPTK_PREC_MIXED;
+    if (num_eigen == 1) {
+      if (mixed) normals_launch<D, PPTK_PREC_MIXED, 1>(t, k, r2, dr, m, evecs, evals, sizes, c, s);
+      else normals_launch<D, PPTK_PREC_FP64, 1>(t, k, r2, dr, m, evecs, evals, sizes, c, s);
+    } else {
+      if (mixed) normals_launch<D, PPTK_PREC_MIXED, 3>(t, k, r2, dr, m, evecs, evals, sizes, c, s);
+      else normals_launch<D, PPTK_PREC_FP64, 3>(t, k, r2, dr, m, evecs, evals, sizes, c, s);
+    }
+  }
+  oe.finish(s);
+  ov.finish(s);
+  os.finish(s);
+  unsigned long long nf = 0;
+  PK_HIP(hipMemcpyAsync(&nf, nfb.get(), 8, hipMemcpyDeviceToHost, s));
+  PK_HIP(hipStreamSynchronize(s));
+  if (out_num_fp64) *out_num_fp64 = (int64_t)nf;
+}
+
+}  // namespace
+}  // namespace pk
+
+extern "C" pptk_status pptk_normals(const pptk_tree* tree, int k, double r, const int32_t* rows,
+                                    int64_t m, int num_eigen, pptk_precision precision,
+                                    void* out_evecs, void* out_evals, int32_t* out_sizes,
+                                    int64_t* out_num_fp64, pptk_mem mem, void* stream) {
+  return pk::guard([&] {
+    pk::check_tree(tree);
+    if (tree->dim != 3) pk::fail(PPTK_E_VALUE, "normals need 3-D points (tree dim = %d)", tree->dim);
+    if (tree->dtype != PPTK_F32 && tree->dtype != PPTK_F64)
+      pk::fail(PPTK_E_TYPE, "points must be float32 or float64");
+    if (!(r > 0.0)) pk::fail(PPTK_E_VALUE, "r must be positive");
+    if (k == 0) pk::fail(PPTK_E_VALUE, "k cannot be zero");
+    if (k < 0 && r == __builtin_huge_val()) pk::fail(PPTK_E_VALUE, "invalid combo: r == inf and k < 0");
+    if (num_eigen != 1 && num_eigen != 3) pk::fail(PPTK_E_VALUE, "num_eigen must be 1 or 3");
+    if (rows && m < 0) pk::fail(PPTK_E_VALUE, "negative subsample size");
+    if (!out_evecs) pk::fail(PPTK_E_VALUE, "null eigenvector output");
+    pptk_precision prec = precision;
+    if (prec == PPTK_PREC_AUTO) prec = tree->dtype == PPTK_F64 ? PPTK_PREC_FP64 : PPTK_PREC_MIXED;
+    if (prec != PPTK_PREC_MIXED && prec != PPTK_PREC_FP64)
+      pk::fail(PPTK_E_VALUE, "unknown precision mode %d", (int)precision);
+    hipStream_t s = pk::pick_stream(stream);
+    if (tree->dist == PPTK_F64)
+      pk::normals_run<double>(tree, k, r, rows, m, num_eigen, prec, out_evecs, out_evals,
+                              out_sizes, out_num_fp64, mem, s);
+    else
+      pk::normals_run<float>(tree, k, r, rows, m, num_eigen, prec, out_evecs, out_evals,
+                             out_sizes, out_num_fp64, mem, s);
+  });
+}

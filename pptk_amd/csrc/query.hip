@@ -1,0 +1,383 @@
+// kNN and r-near queries on a device-resident tree.
+//
+// kNN  (pptk_knn)         : one lane per query, stackless near-first search,
+//                           top-k in registers (k <= 32) or in a global-memory
+//                           heap (k > 32).
+// r-near (pptk_rnear_*)   : count pass -> exclusive scan -> fill pass into
+//                           CSR (int64 offsets, int32 original indices).
+// Reference entry points: KdTree::KNearestNeighbors[Self] and
+// RNearNeighbors[Self] (pptk/kdtree/src/kdtree-impl.h:873-1069).
+#include <algorithm>
+#include <vector>
+
+#include "query.hpp"
+#include "sort.hpp"
+#include "traverse.hpp"
+#include "tree.hpp"
+
+namespace pk {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+template <typename D, int DIM, int K>
+__global__ __launch_bounds__(kBlock) void knn_kernel(TreeView<D, DIM> tv, int mode,
+                                                     const int32_t* __restrict__ qlist,
+                                                     const D* __restrict__ qpts, int64_t m,
+                                                     int64_t nrows, int k, D r2,
+                                                     int32_t* __restrict__ out_idx,
+                                                     int32_t* __restrict__ out_cnt,
+                                                     D* __restrict__ out_dist) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= m) return;
+  D q[DIM];
+  int32_t excl;
+  int64_t row;
+  bool ok;
+  if (!resolve_query<D, DIM>(tv, mode, qlist, qpts, t, nrows, q, excl, row, ok)) return;
+  TopK<D, K, false> st;
+  st.init(k);
+  if (ok) knn_search<D, DIM>(tv, q, r2, excl, st);
+  int32_t c = 0;
+  int32_t* orow = out_idx + row * k;
+#pragma unroll
+  for (int s = 0; s < K; ++s) {
+    if (s >= K - k) {
+      const bool f = st.d[s] < pos_inf<D>();
+      orow[s - (K - k)] = f ? st.o[s] : -1;
+      if (out_dist) out_dist[row * k + (s - (K - k))] = f ? st.d[s] : pos_inf<D>();
+      c += f ? 1 : 0;
+    }
+  }
+  out_cnt[row] = c;
+}
+
+template <typename D, int DIM>
+__global__ __launch_bounds__(kBlock) void knn_large_kernel(TreeView<D, DIM> tv, int mode,
+                                                           const int32_t* __restrict__ qlist,
+                                                           const D* __restrict__ qpts, int64_t t0,
+                                                           int64_t t1, int64_t nrows, int k, D r2,
+                                                           int32_t* __restrict__ out_idx,
+                                                           int32_t* __restrict__ out_cnt,
+                                                           D* __restrict__ ws_d,
+                                                           D* __restrict__ out_dist) {
+  const int64_t t = t0 + (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= t1) return;
+  D q[DIM];
+  int32_t excl;
+  int64_t row;
+  bool ok;
+  if (!resolve_query<D, DIM>(tv, mode, qlist, qpts, t, nrows, q, excl, row, ok)) return;
+  D* hd = ws_d + (t - t0) * k;
+  int32_t* ho = out_idx + row * k;
+  HeapK<D> st;
+  st.init(hd, ho, nullptr, k);
+  if (ok) knn_search<D, DIM>(tv, q, r2, excl, st);
+  st.finish();
+  for (int s = st.size; s < k; ++s) ho[s] = -1;
+  out_cnt[row] = st.size;
+  if (out_dist)
+    for (int s = 0; s < k; ++s) out_dist[row * k + s] = s < st.size ? hd[s] : pos_inf<D>();
+}
+
+template <typename D, int DIM, bool FILL>
+__global__ __launch_bounds__(kBlock) void rnear_kernel(TreeView<D, DIM> tv, int mode,
+                                                       const int32_t* __restrict__ qlist,
+                                                       const D* __restrict__ qpts, int64_t m,
+                                                       int64_t nrows, D r2,
+                                                       int32_t* __restrict__ counts,
+                                                       const int64_t* __restrict__ offsets,
+                                                       int32_t* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= m) return;
+  D q[DIM];
+  int32_t excl;
+  int64_t row;
+  bool ok;
+  if (!resolve_query<D, DIM>(tv, mode, qlist, qpts, t, nrows, q, excl, row, ok)) return;
+  if constexpr (FILL) {
+    if (!ok) return;
+    const int64_t o0 = offsets[row];
+    rnear_walk<D, DIM, true>(tv, q, r2, excl, out + o0, offsets[row + 1] - o0);
+  } else {
+    counts[row] = ok ? (int32_t)rnear_walk<D, DIM, false>(tv, q, r2, excl, nullptr, 0) : 0;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void fill_kernel(T* __restrict__ p, int64_t n, T v) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+inline unsigned grid_of(int64_t n) { return (unsigned)std::max<int64_t>(1, ceil_div(n, kBlock)); }
+
+}  // namespace
+
+template <typename T>
+void fill_async(T* p, int64_t n, T v, hipStream_t s) {
+  if (n <= 0) return;
+  fill_kernel<T><<<grid_of(n), kBlock, 0, s>>>(p, n, v);
+  PK_LAUNCH_CHECK();
+}
+template void fill_async<int32_t>(int32_t*, int64_t, int32_t, hipStream_t);
+template void fill_async<float>(float*, int64_t, float, hipStream_t);
+template void fill_async<double>(double*, int64_t, double, hipStream_t);
+
+template <typename D>
+void prepare_queries(const pptk_tree* t, const pptk_queries* q, hipStream_t s, PreparedQueries& pq) {
+  if (!q) fail(PPTK_E_VALUE, "null query set");
+  switch (q->kind) {
+    case PPTK_Q_ALL:
+      pq.mode = kQueryAll;
+      pq.m = t->n_valid;
+      pq.nrows = t->n_valid;
+      break;
+    case PPTK_Q_INDICES: {
+      pq.mode = kQueryList;
+      if (q->m < 0) fail(PPTK_E_VALUE, "negative query count");
+      pq.m = pq.nrows = q->m;
+      if (q->m > 0 && !q->indices) fail(PPTK_E_VALUE, "null query indices");
+      if (q->mem == PPTK_MEM_HOST) {
+        for (int64_t i = 0; i < q->m; ++i)
+          if (q->indices[i] < 0 || q->indices[i] >= t->n_input)
+            fail(PPTK_E_RUNTIME, "%lld-th query index %d is outside of [0,%lld]", (long long)i,
+                 (int)q->indices[i], (long long)t->n_input - 1);
+      }
+      stage_input(pq.list, q->indices, (size_t)q->m * 4, q->mem, s);
+      pq.dlist = (const int32_t*)pq.list.ptr;
+      break;
+    }
+    case PPTK_Q_POINTS: {
+      pq.mode = kQueryPoints;
+      if (q->m < 0) fail(PPTK_E_VALUE, "negative query count");
+      pq.m = pq.nrows = q->m;
+      if (q->m > 0 && !q->points) fail(PPTK_E_VALUE, "null query points");
+      if (!dtype_valid(q->dtype)) fail(PPTK_E_TYPE, "unsupported query dtype %d", (int)q->dtype);
+      const size_t cnt = (size_t)q->m * t->dim;
+      if (q->mem == PPTK_MEM_DEVICE) {
+        if (q->dtype != t->dist)
+          fail(PPTK_E_TYPE, "device query points must have the tree's distance dtype");
+        pq.dpts = q->points;
+      } else if (cnt > 0) {
+        std::vector<D> conv(cnt);
+        D* dst = conv.data();
+        dispatch_dtype(q->dtype, [&](auto tag) {
+          using T = typename decltype(tag)::type;
+          const T* src = (const T*)q->points;
+          for (size_t i = 0; i < cnt; ++i) dst[i] = (D)src[i];
+        });
+        pq.conv.reset(cnt * sizeof(D));
+        PK_HIP(hipMemcpyAsync(pq.conv.get(), conv.data(), cnt * sizeof(D), hipMemcpyHostToDevice, s));
+        PK_HIP(hipStreamSynchronize(s));  // `conv` goes out of scope
+        pq.dpts = pq.conv.get();
+      }
+      break;
+    }
+    default:
+      fail(PPTK_E_VALUE, "unknown query kind %d", (int)q->kind);
+  }
+}
+template void prepare_queries<float>(const pptk_tree*, const pptk_queries*, hipStream_t, PreparedQueries&);
+template void prepare_queries<double>(const pptk_tree*, const pptk_queries*, hipStream_t, PreparedQueries&);
+
+namespace {
+
+template <typename D, int DIM>
+void knn_run(const pptk_tree* t, const pptk_queries* qs, int k, double dmax, int32_t* out_idx,
+             int32_t* out_cnt, void* out_dist, pptk_mem out_mem, hipStream_t s) {
+  PreparedQueries pq;
+  prepare_queries<D>(t, qs, s, pq);
+  const D r = (D)dmax;
+  const D r2 = r * r;
+  const int64_t nrows = pq.nrows;
+  DevOutput oi, oc, od;
+  oi.init(out_idx, (size_t)nrows * k * 4, out_mem);
+  oc.init(out_cnt, (size_t)nrows * 4, out_mem);
+  od.init(out_dist, (size_t)nrows * k * sizeof(D), out_mem);
+  if (nrows == 0) return;
+  const TreeView<D, DIM> tv = tree_view<D, DIM>(t);
+  if (pq.mode == kQueryAll && t->n_valid < t->n_input) {
+    // Rows of NaN/inf points receive no thread; pre-fill them empty.
+    fill_async<int32_t>((int32_t*)oi.dptr, nrows * k, -1, s);
+    fill_async<int32_t>((int32_t*)oc.dptr, nrows, 0, s);
+    if (od.dptr) fill_async<D>((D*)od.dptr, nrows * k, pos_inf<D>(), s);
+  }
+  const int64_t m = pq.m;
+  const D* qp = (const D*)pq.dpts;
+  if (m > 0) {
+    if (k <= 32) {
+      auto launch = [&](auto kc) {
+        constexpr int K = decltype(kc)::value;
+        knn_kernel<D, DIM, K><<<grid_of(m), kBlock, 0, s>>>(
+            tv, pq.mode, pq.dlist, qp, m, nrows, k, r2, (int32_t*)oi.dptr, (int32_t*)oc.dptr,
+            (D*)od.dptr);
+        PK_LAUNCH_CHECK();
+      };
+      if (k <= 4) launch(std::integral_constant<int, 4>());
+      else if (k <= 8) launch(std::integral_constant<int, 8>());
+      else if (k <= 16) launch(std::integral_constant<int, 16>());
+      else launch(std::integral_constant<int, 32>());
+    } else {
+      // Global-memory heaps; bound the distance workspace to ~256 MB.
+      const int64_t chunk = std::max<int64_t>(kBlock, ((int64_t)256 << 20) / ((int64_t)k * sizeof(D)));
+      DevBuf ws((size_t)std::min(chunk, m) * k * sizeof(D));
+      for (int64_t t0 = 0; t0 < m; t0 += chunk) {
+        const int64_t t1 = std::min(m, t0 + chunk);
+        knn_large_kernel<D, DIM><<<grid_of(t1 - t0), kBlock, 0, s>>>(
+            tv, pq.mode, pq.dlist, qp, t0, t1, nrows, k, r2, (int32_t*)oi.dptr,
+            (int32_t*)oc.dptr, ws.as<D>(), (D*)od.dptr);
+        PK_LAUNCH_CHECK();
+      }
+      PK_HIP(hipStreamSynchronize(s));
+    }
+  }
+  oi.finish(s);
+  oc.finish(s);
+  od.finish(s);
+  PK_HIP(hipStreamSynchronize(s));
+}
+
+template <typename D, int DIM>
+void rnear_count_run(const pptk_tree* t, const pptk_queries* qs, double dmax, int64_t* out_offsets,
+                     int64_t* out_total, pptk_mem out_mem, hipStream_t s) {
+  PreparedQueries pq;
+  prepare_queries<D>(t, qs, s, pq);
+  const D r = (D)dmax;
+  const D r2 = r * r;
+  const int64_t nrows = pq.nrows;
+  DevOutput oo;
+  oo.init(out_offsets, (size_t)(nrows + 1) * 8, out_mem);
+  DevBuf counts((size_t)std::max<int64_t>(nrows, 1) * 4);
+  if (nrows > 0) PK_HIP(hipMemsetAsync(counts.get(), 0, (size_t)nrows * 4, s));
+  const TreeView<D, DIM> tv = tree_view<D, DIM>(t);
+  if (pq.m > 0) {
+    rnear_kernel<D, DIM, false><<<grid_of(pq.m), kBlock, 0, s>>>(
+        tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.m, nrows, r2, counts.as<int32_t>(),
+        nullptr, nullptr);
+    PK_LAUNCH_CHECK();
+  }
+  int64_t* doff = (int64_t*)oo.dptr;
+  DevBuf tmp;
+  if (!doff) {
+    tmp.reset((size_t)(nrows + 1) * 8);
+    doff = tmp.as<int64_t>();
+  }
+  exclusive_scan<int32_t, int64_t>(counts.as<int32_t>(), doff, nrows, s);
+  int64_t total = 0;
+  PK_HIP(hipMemcpyAsync(&total, doff + nrows, 8, hipMemcpyDeviceToHost, s));
+  oo.finish(s);
+  PK_HIP(hipStreamSynchronize(s));
+  if (out_total) *out_total = total;
+}
+
+template <typename D, int DIM>
+void rnear_fill_run(const pptk_tree* t, const pptk_queries* qs, double dmax, const int64_t* offsets,
+                    int32_t* out_idx, pptk_mem out_mem, hipStream_t s) {
+  PreparedQueries pq;
+  prepare_queries<D>(t, qs, s, pq);
+  const D r = (D)dmax;
+  const D r2 = r * r;
+  const int64_t nrows = pq.nrows;
+  if (!offsets) fail(PPTK_E_VALUE, "null offsets");
+  int64_t total = 0;
+  if (out_mem == PPTK_MEM_HOST) total = offsets[nrows];
+  else PK_HIP(hipMemcpyAsync(&total, offsets + nrows, 8, hipMemcpyDeviceToHost, s));
+  PK_HIP(hipStreamSynchronize(s));
+  if (total < 0) fail(PPTK_E_VALUE, "negative total in offsets");
+  DevInput doff;
+  stage_input(doff, offsets, (size_t)(nrows + 1) * 8, out_mem, s);
+  DevOutput oi;
+  oi.init(out_idx, (size_t)total * 4, out_mem);
+  if (total > 0 && !out_idx) fail(PPTK_E_VALUE, "null output");
+  const TreeView<D, DIM> tv = tree_view<D, DIM>(t);
+  if (pq.m > 0 && total > 0) {
+    rnear_kernel<D, DIM, true><<<grid_of(pq.m), kBlock, 0, s>>>(
+        tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.m, nrows, r2, nullptr,
+        (const int64_t*)doff.ptr, (int32_t*)oi.dptr);
+    PK_LAUNCH_CHECK();
+  }
+  oi.finish(s);
+  PK_HIP(hipStreamSynchronize(s));
+}
+
+void check_dmax(double dmax) {
+  if (!(dmax >= 0.0)) fail(PPTK_E_VALUE, "dmax must be non-negative (got %g)", dmax);
+}
+
+}  // namespace
+}  // namespace pk
+
+namespace pk {
+pptk_status api_knn(const pptk_tree* tree, const pptk_queries* q, int k, double dmax,
+                                int32_t* out_idx, int32_t* out_cnt, void* out_dist,
+                                pptk_mem out_mem, void* stream) {
+  return pk::guard([&] {
+    pk::check_tree(tree);
+    if (k < 1) pk::fail(PPTK_E_VALUE, "k must be positive (got %d)", k);
+    pk::check_dmax(dmax);
+    hipStream_t s = pk::pick_stream(stream);
+    auto run = [&](auto dt, auto dimc) {
+      using D = typename decltype(dt)::type;
+      constexpr int DIM = decltype(dimc)::value;
+      pk::knn_run<D, DIM>(tree, q, k, dmax, out_idx, out_cnt, out_dist, out_mem, s);
+    };
+    pk::dispatch_tree(tree, run);
+  });
+}
+}  // namespace pk
+
+extern "C" pptk_status pptk_knn(const pptk_tree* tree, const pptk_queries* q, int k, double dmax,
+                                int32_t* out_idx, int32_t* out_cnt, void* out_dist,
+                                pptk_mem out_mem, void* stream) {
+  return pk::api_knn(tree, q, k, dmax, out_idx, out_cnt, out_dist, out_mem, stream);
+}
+
+namespace pk {
+pptk_status api_rnear_count(const pptk_tree* tree, const pptk_queries* q, double dmax,
+                                        int64_t* out_offsets, int64_t* out_total,
+                                        pptk_mem out_mem, void* stream) {
+  return pk::guard([&] {
+    pk::check_tree(tree);
+    pk::check_dmax(dmax);
+    hipStream_t s = pk::pick_stream(stream);
+    auto run = [&](auto dt, auto dimc) {
+      using D = typename decltype(dt)::type;
+      constexpr int DIM = decltype(dimc)::value;
+      pk::rnear_count_run<D, DIM>(tree, q, dmax, out_offsets, out_total, out_mem, s);
+    };
+    pk::dispatch_tree(tree, run);
+  });
+}
+}  // namespace pk
+
+extern "C" pptk_status pptk_rnear_count(const pptk_tree* tree, const pptk_queries* q, double dmax,
+                                        int64_t* out_offsets, int64_t* out_total,
+                                        pptk_mem out_mem, void* stream) {
+  return pk::api_rnear_count(tree, q, dmax, out_offsets, out_total, out_mem, stream);
+}
+
+namespace pk {
+pptk_status api_rnear_fill(const pptk_tree* tree, const pptk_queries* q, double dmax,
+                                       const int64_t* offsets, int32_t* out_idx, pptk_mem out_mem,
+                                       void* stream) {
+  return pk::guard([&] {
+    pk::check_tree(tree);
+    pk::check_dmax(dmax);
+    hipStream_t s = pk::pick_stream(stream);
+    auto run = [&](auto dt, auto dimc) {
+      using D = typename decltype(dt)::type;
+      constexpr int DIM = decltype(dimc)::value;
+      pk::rnear_fill_run<D, DIM>(tree, q, dmax, offsets, out_idx, out_mem, s);
+    };
+    pk::dispatch_tree(tree, run);
+  });
+}
+}  // namespace pk
+
+extern "C" pptk_status pptk_rnear_fill(const pptk_tree* tree, const pptk_queries* q, double dmax,
+                                       const int64_t* offsets, int32_t* out_idx, pptk_mem out_mem,
+                                       void* stream) {
+  return pk::api_rnear_fill(tree, q, dmax, offsets, out_idx, out_mem, stream);
+}

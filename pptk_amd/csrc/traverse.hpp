@@ -1,0 +1,330 @@
+// Per-lane tree traversal shared by the kNN, r-near and normals kernels.
+//
+// The tree is an implicit complete binary tree (node i has children 2i+1 and
+// 2i+2, leaf j is node L-1+j and holds tree positions [j*B, (j+1)*B)), so a
+// lane walks it without a stack: it keeps its node, its depth and one bit per
+// level recording which child it entered first. Backtracking reads the
+// sibling from that bit and re-tests it against the current k-th bound.
+//
+// Neighbour order is the key (d, original index): smaller squared distance
+// first, ties broken by the smaller original index. A node is entered while
+// its box lower bound is < r*r and <= the current k-th distance (<=, because
+// a point at exactly the k-th distance with a smaller index still enters).
+// Reference semantics: pointkd::impl::KNearestNeighborsHelper
+// (pptk/kdtree/src/kdtree-impl.h:348-429) and RNearNeighborsHelper
+// (kdtree-impl.h:519-558).
+#pragma once
+#include "common.hpp"
+
+namespace pk {
+
+enum QueryMode : int { kQueryAll = 0, kQueryList = 1, kQueryPoints = 2 };
+
+template <typename D>
+__device__ __forceinline__ bool key_less(D a, int32_t ao, D b, int32_t bo) {
+  return a < b || (a == b && ao < bo);
+}
+
+// Sorted top-k in registers, K slots, runtime k <= K. Slots [0, K-k) hold
+// -inf sentinels that never move, so the k-th entry is always slot K-1 and no
+// register array is indexed dynamically.
+template <typename D, int K, bool POS>
+struct TopK {
+  D d[K];
+  int32_t o[K];
+  int32_t p[POS ? K : 1];
+
+  __device__ __forceinline__ void init(int k) {
+#pragma unroll
+    for (int s = 0; s < K; ++s) {
+      const bool pad = s < K - k;
+      d[s] = pad ? -pos_inf<D>() : pos_inf<D>();
+      o[s] = pad ? INT32_MIN : INT32_MAX;
+      if constexpr (POS) p[s] = -1;
+    }
+  }
+  __device__ __forceinline__ D bound() const { return d[K - 1]; }
+  __device__ __forceinline__ bool beats(D dd, int32_t oo) const {
+    return key_less(dd, oo, d[K - 1], o[K - 1]);
+  }
+  // Precondition: beats(dd, oo).
+  __device__ __forceinline__ void insert(D dd, int32_t oo, int32_t pp) {
+#pragma unroll
+    for (int s = K - 1; s > 0; --s) {
+      const bool sh = key_less(dd, oo, d[s - 1], o[s - 1]);
+      const bool here = key_less(dd, oo, d[s], o[s]);
+      d[s] = sh ? d[s - 1] : (here ? dd : d[s]);
+      o[s] = sh ? o[s - 1] : (here ? oo : o[s]);
+      if constexpr (POS) p[s] = sh ? p[s - 1] : (here ? pp : p[s]);
+    }
+    const bool h0 = key_less(dd, oo, d[0], o[0]);
+    d[0] = h0 ? dd : d[0];
+    o[0] = h0 ? oo : o[0];
+    if constexpr (POS) p[0] = h0 ? pp : p[0];
+  }
+};
+
+// Binary max-heap on (d, o) in global memory, for k above the register
+// limit. `hd`, `ho`, `hp` point at this query's k-slot row.
+template <typename D>
+struct HeapK {
+  D* hd;
+  int32_t* ho;
+  int32_t* hp;  // may be null
+  int k;
+  int size;
+  D top_d;
+  int32_t top_o;
+
+  __device__ __forceinline__ void init(D* d_, int32_t* o_, int32_t* p_, int k_) {
+    hd = d_; ho = o_; hp = p_; k = k_; size = 0;
+    top_d = pos_inf<D>(); top_o = INT32_MAX;
+  }
+  __device__ __forceinline__ D bound() const { return size < k ? pos_inf<D>() : top_d; }
+  __device__ __forceinline__ bool beats(D dd, int32_t oo) const {
+    return size < k || key_less(dd, oo, top_d, top_o);
+  }
+  __device__ __forceinline__ void swap_(int a, int b) {
+    const D td = hd[a]; hd[a] = hd[b]; hd[b] = td;
+    const int32_t to = ho[a]; ho[a] = ho[b]; ho[b] = to;
+    if (hp) { const int32_t tp = hp[a]; hp[a] = hp[b]; hp[b] = tp; }
+  }
+  __device__ void sift_down(int i, int n) {
+    for (;;) {
+      const int l = 2 * i + 1;
+      if (l >= n) return;
+      int c = l;
+      if (l + 1 < n && key_less(hd[l], ho[l], hd[l + 1], ho[l + 1])) c = l + 1;
+      if (!key_less(hd[i], ho[i], hd[c], ho[c])) return;
+      swap_(i, c);
+      i = c;
+    }
+  }
+  __device__ void insert(D dd, int32_t oo, int32_t pp) {
+    if (size < k) {
+      int i = size++;
+      hd[i] = dd; ho[i] = oo;
+      if (hp) hp[i] = pp;
+      while (i > 0) {
+        const int par = (i - 1) >> 1;
+        if (!key_less(hd[par], ho[par], hd[i], ho[i])) break;
+        swap_(par, i);
+        i = par;
+      }
+    } else {
+      hd[0] = dd; ho[0] = oo;
+      if (hp) hp[0] = pp;
+      sift_down(0, size);
+    }
+    top_d = hd[0];
+    top_o = ho[0];
+  }
+  // Heapsort in place: slots [0, size) end up ascending.
+  __device__ void finish() {
+    for (int e = size - 1; e > 0; --e) {
+      swap_(0, e);
+      sift_down(0, e);
+    }
+  }
+};
+
+template <typename D, int DIM, typename State>
+__device__ __forceinline__ void knn_scan_leaf(const TreeView<D, DIM>& tv, int32_t j, const D* q,
+                                              D r2, int32_t excl, State& st) {
+  const int32_t base = j * tv.leaf;
+  const int32_t rem = tv.n - base;
+  const int32_t cnt = rem < tv.leaf ? rem : tv.leaf;
+  const D kb = st.bound();
+  uint64_t mask = 0;
+  for (int32_t s = 0; s < cnt; ++s) {
+    const PointRec<D, DIM> r = tv.pts[base + s];
+    const D dd = dist2<D, DIM>(q, r.c);
+    const bool pass = (dd < r2) && (dd <= kb) && (base + s != excl);
+    mask |= (uint64_t)pass << s;
+  }
+  while (mask) {
+    const int s = __builtin_ctzll(mask);
+    mask &= mask - 1ull;
+    const PointRec<D, DIM> r = tv.pts[base + s];
+    const D dd = dist2<D, DIM>(q, r.c);
+    if (st.beats(dd, r.orig)) st.insert(dd, r.orig, base + s);
+  }
+}
+
+// k-nearest search from the root, near child first.
+template <typename D, int DIM, typename State>
+__device__ void knn_search(const TreeView<D, DIM>& tv, const D* q, D r2, int32_t excl, State& st) {
+  const int32_t leaf0 = tv.L - 1;
+  if (!(box_min_dist2<D, DIM>(tv.box[0], q) < r2)) return;
+  int32_t node = 0;
+  int depth = 0;
+  uint32_t flip = 0;
+  for (;;) {
+    bool at_leaf = true;
+    while (node < leaf0) {
+      const int32_t c0 = 2 * node + 1;
+      const D m0 = box_min_dist2<D, DIM>(tv.box[c0], q);
+      const D m1 = box_min_dist2<D, DIM>(tv.box[c0 + 1], q);
+      const D kb = st.bound();
+      const bool n0 = (m0 < r2) && (m0 <= kb);
+      const bool n1 = (m1 < r2) && (m1 <= kb);
+      if (!(n0 || n1)) { at_leaf = false; break; }
+      const bool right = n1 && (!n0 || m1 < m0);
+      const uint32_t bit = 1u << depth;
+      flip = right ? (flip | bit) : (flip & ~bit);
+      node = c0 + (right ? 1 : 0);
+      ++depth;
+    }
+    if (at_leaf) knn_scan_leaf<D, DIM>(tv, node - leaf0, q, r2, excl, st);
+    for (;;) {
+      if (depth == 0) return;
+      const bool is_right = (node & 1) == 0;
+      const bool first_right = (flip >> (depth - 1)) & 1u;
+      if (is_right == first_right) {
+        const int32_t sib = is_right ? node - 1 : node + 1;
+        const D ms = box_min_dist2<D, DIM>(tv.box[sib], q);
+        if (ms < r2 && ms <= st.bound()) { node = sib; break; }
+      }
+      node = (node - 1) >> 1;
+      --depth;
+    }
+  }
+}
+
+// r-near walk in tree order (left subtree first). Counts (FILL = false) or
+// writes at most `cap` original indices to `out` (FILL = true). A subtree
+// whose box upper bound is < r*r is emitted whole, as
+// RNearNeighborsHelper does (kdtree-impl.h:527-529).
+template <typename D, int DIM, bool FILL>
+__device__ int64_t rnear_walk(const TreeView<D, DIM>& tv, const D* q, D r2, int32_t excl,
+                              int32_t* __restrict__ out, int64_t cap) {
+  int64_t cnt = 0;
+  const int32_t leaf0 = tv.L - 1;
+  int32_t node = 0;
+  for (;;) {
+    bool descend = false;
+    const BoxRec<D, DIM> b = tv.box[node];
+    if (box_min_dist2<D, DIM>(b, q) < r2) {
+      if (box_max_dist2<D, DIM>(b, q) < r2) {
+        const int dd = 31 - __clz(node + 1);
+        const int sh = tv.depth - dd;
+        const int64_t j0 = ((int64_t)(node + 1) << sh) - (int64_t)tv.L;
+        const int64_t s0 = j0 * tv.leaf;
+        int64_t s1 = (j0 + ((int64_t)1 << sh)) * tv.leaf;
+        if (s1 > tv.n) s1 = tv.n;
+        if constexpr (FILL) {
+          for (int64_t p = s0; p < s1; ++p) {
+            if (p == excl) continue;
+            if (cnt < cap) out[cnt] = tv.orig[p];
+            ++cnt;
+          }
+        } else {
+          cnt += (s1 - s0) - ((excl >= s0 && excl < s1) ? 1 : 0);
+        }
+      } else if (node >= leaf0) {
+        const int32_t base = (node - leaf0) * tv.leaf;
+        const int32_t rem = tv.n - base;
+        const int32_t c = rem < tv.leaf ? rem : tv.leaf;
+        for (int32_t s = 0; s < c; ++s) {
+          const PointRec<D, DIM> r = tv.pts[base + s];
+          if (dist2<D, DIM>(q, r.c) < r2 && base + s != excl) {
+            if constexpr (FILL) {
+              if (cnt < cap) out[cnt] = r.orig;
+            }
+            ++cnt;
+          }
+        }
+      } else {
+        node = 2 * node + 1;
+        descend = true;
+      }
+    }
+    if (descend) continue;
+    while (node != 0 && (node & 1) == 0) node = (node - 1) >> 1;
+    if (node == 0) break;
+    node = node + 1;
+  }
+  return cnt;
+}
+
+// Calls visit(record) for every point with d < r*r, in tree order (used by
+// the r-near normals; the query point itself is included).
+template <typename D, int DIM, typename V>
+__device__ void rnear_visit(const TreeView<D, DIM>& tv, const D* q, D r2, V& visit) {
+  const int32_t leaf0 = tv.L - 1;
+  int32_t node = 0;
+  for (;;) {
+    bool descend = false;
+    const BoxRec<D, DIM> b = tv.box[node];
+    if (box_min_dist2<D, DIM>(b, q) < r2) {
+      if (box_max_dist2<D, DIM>(b, q) < r2) {
+        const int dd = 31 - __clz(node + 1);
+        const int sh = tv.depth - dd;
+        const int64_t j0 = ((int64_t)(node + 1) << sh) - (int64_t)tv.L;
+        const int64_t s0 = j0 * tv.leaf;
+        int64_t s1 = (j0 + ((int64_t)1 << sh)) * tv.leaf;
+        if (s1 > tv.n) s1 = tv.n;
+        for (int64_t p = s0; p < s1; ++p) visit(tv.pts[p]);
+      } else if (node >= leaf0) {
+        const int32_t base = (node - leaf0) * tv.leaf;
+        const int32_t rem = tv.n - base;
+        const int32_t c = rem < tv.leaf ? rem : tv.leaf;
+        for (int32_t s = 0; s < c; ++s) {
+          const PointRec<D, DIM> r = tv.pts[base + s];
+          if (dist2<D, DIM>(q, r.c) < r2) visit(r);
+        }
+      } else {
+        node = 2 * node + 1;
+        descend = true;
+      }
+    }
+    if (descend) continue;
+    while (node != 0 && (node & 1) == 0) node = (node - 1) >> 1;
+    if (node == 0) break;
+    node = node + 1;
+  }
+}
+
+// Resolves the query of thread `t`. Returns false if the thread has no output
+// row (ALL mode, original index >= nrows). Sets ok = false for an invalid
+// query (NaN/inf point, index of an invalid row, index out of range), which
+// yields an empty result.
+template <typename D, int DIM>
+__device__ __forceinline__ bool resolve_query(const TreeView<D, DIM>& tv, int mode,
+                                              const int32_t* __restrict__ qlist,
+                                              const D* __restrict__ qpts, int64_t t,
+                                              int64_t nrows, D* q, int32_t& excl, int64_t& row,
+                                              bool& ok) {
+  ok = true;
+  excl = -1;
+  if (mode == kQueryAll) {
+    const PointRec<D, DIM> r = tv.pts[t];
+    row = r.orig;
+    if (row >= nrows) return false;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) q[d] = r.c[d];
+    excl = (int32_t)t;
+  } else if (mode == kQueryList) {
+    row = t;
+    const int32_t o = qlist[t];
+    const int32_t p = (o >= 0 && (int64_t)o < tv.n_input) ? tv.fwd[o] : -1;
+    if (p < 0) {
+      ok = false;
+    } else {
+      const PointRec<D, DIM> r = tv.pts[p];
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) q[d] = r.c[d];
+      excl = p;
+    }
+  } else {
+    row = t;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      q[d] = qpts[t * DIM + d];
+      ok = ok && finite_val(q[d]);
+    }
+  }
+  return true;
+}
+
+}  // namespace pk

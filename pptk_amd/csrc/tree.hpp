@@ -1,0 +1,63 @@
+#pragma once
+#include "common.hpp"
+
+// Host-side tree handle (opaque `pptk_tree` of the C ABI).
+struct pptk_tree {
+  static constexpr uint64_t kMagic = 0x70707462676b6474ull;
+  uint64_t magic = kMagic;
+  int dim = 3;
+  pptk_dtype dtype = PPTK_F32;
+  pptk_dtype dist = PPTK_F32;
+  int64_t n_input = 0;
+  int64_t n_valid = 0;
+  int leaf = 16;
+  int depth = 0;
+  int64_t nleaves = 0;
+  int64_t L = 1;
+  double lo[4] = {0, 0, 0, 0};
+  double hi[4] = {0, 0, 0, 0};
+  pk::DevBuf pts;   // PointRec<D,dim>[n_valid]
+  pk::DevBuf orig;  // int32[n_valid]
+  pk::DevBuf fwd;   // int32[n_input]
+  pk::DevBuf box;   // BoxRec<D,dim>[2L-1]
+};
+
+namespace pk {
+
+const pptk_tree* check_tree(const pptk_tree* t);
+
+template <typename D, int DIM>
+TreeView<D, DIM> tree_view(const pptk_tree* t) {
+  TreeView<D, DIM> v;
+  v.pts = t->pts.as<const PointRec<D, DIM>>();
+  v.orig = t->orig.as<const int32_t>();
+  v.fwd = t->fwd.as<const int32_t>();
+  v.box = t->box.as<const BoxRec<D, DIM>>();
+  v.n = (int32_t)t->n_valid;
+  v.leaf = t->leaf;
+  v.L = (int32_t)t->L;
+  v.depth = t->depth;
+  v.n_input = t->n_input;
+  return v;
+}
+
+// Calls f(Tag<D>{}, integral_constant<int, DIM>{}) for the tree's distance
+// type D and dimension.
+template <typename F>
+void dispatch_tree(const pptk_tree* t, F&& f) {
+  const bool dbl = t->dist == PPTK_F64;
+  using I2 = std::integral_constant<int, 2>;
+  using I3 = std::integral_constant<int, 3>;
+  using I4 = std::integral_constant<int, 4>;
+  switch (t->dim) {
+    case 2: if (dbl) f(Tag<double>{}, I2{}); else f(Tag<float>{}, I2{}); break;
+    case 3: if (dbl) f(Tag<double>{}, I3{}); else f(Tag<float>{}, I3{}); break;
+    case 4: if (dbl) f(Tag<double>{}, I4{}); else f(Tag<float>{}, I4{}); break;
+    default: fail(PPTK_E_RUNTIME, "invalid k-d tree dim = %d", t->dim);
+  }
+}
+
+// Tuned default bucket size.
+int default_leaf_size(int dim, bool dbl, int64_t n);
+
+}  // namespace pk

@@ -1,0 +1,403 @@
+"""GPU parity: the HIP path (through the drop-in modules and the C ABI)
+against the oracle on the same seeded inputs. Index work is compared
+bit-exactly; normals by angle against FP64."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _kd():
+    import pptk_amd
+    return pptk_amd.kdtree
+
+
+def _rows(lst, width=None):
+    """List of int32 arrays -> (m, width) matrix padded with -1, + counts."""
+    cnt = np.array([len(x) for x in lst], np.int32)
+    k = width if width is not None else (int(cnt.max()) if len(cnt) else 1)
+    out = np.full((len(lst), max(k, 1)), -1, np.int32)
+    for i, x in enumerate(lst):
+        assert x.dtype == np.int32 and x.ndim == 1
+        out[i, :len(x)] = x
+    return out, cnt
+
+
+def assert_knn_equal(lst, want, wcnt):
+    got, cnt = _rows(lst, want.shape[1])
+    assert np.array_equal(cnt, wcnt)
+    assert np.array_equal(got, want)
+
+
+# ------------------------------------------------------------ primitives ----
+
+def test_sort_pairs_matches_stable_argsort(gpu):
+    from pptk_amd import capi
+    rng = np.random.default_rng(5)
+    for n, bits in [(1, 63), (1000, 63), (4097, 20), (300_000, 63), (1_000_000, 32)]:
+        keys = rng.integers(0, 2 ** min(bits, 63), n, dtype=np.uint64)
+        keys[: n // 3] = keys[n // 2]  # duplicates exercise stability
+        vals = np.arange(n, dtype=np.uint32)
+        k2, v2 = keys.copy(), vals.copy()
+        capi.check(capi.lib().pptk_sort_pairs_u64(capi._p(k2), capi._p(v2), n, 64,
+                                                  capi.MEM_HOST))
+        order = np.argsort(keys, kind="stable")
+        assert np.array_equal(k2, keys[order]), n
+        assert np.array_equal(v2, vals[order]), n
+
+
+def test_exclusive_scan(gpu):
+    from pptk_amd import capi
+    rng = np.random.default_rng(6)
+    for n in [0, 1, 4095, 4096, 4097, 1_000_003]:
+        x = rng.integers(0, 1000, n).astype(np.int32)
+        out = np.empty(n + 1, np.int64)
+        capi.check(capi.lib().pptk_exclusive_scan_i32(capi._p(x), capi._p(out), n,
+                                                      capi.MEM_HOST))
+        want = np.concatenate([[0], np.cumsum(x, dtype=np.int64)])
+        assert np.array_equal(out, want), n
+
+
+# ----------------------------------------------------------------- build ----
+
+@pytest.mark.parametrize("dim,dtype", [(3, np.float32), (3, np.float64), (2, np.float32),
+                                       (4, np.float64), (3, np.int16)])
+def test_tree_invariants(gpu, dim, dtype):
+    from pptk_amd import capi
+    rng = np.random.default_rng(dim)
+    n = 50_000
+    pts = (rng.random((n, dim)) * 1000).astype(dtype)
+    if np.issubdtype(dtype, np.floating):
+        pts[::97, dim - 1] = np.nan
+        pts[::89, 0] = np.inf
+    t = capi.Tree.build(pts, leaf=13)
+    inf = t.info
+    P, rev, fwd, box = t.export()
+    valid = np.all(np.isfinite(pts.astype(np.float64)), axis=1)
+    assert inf.num_valid == valid.sum()
+    # reverse is a permutation of the valid rows; forward inverts it
+    assert np.array_equal(np.sort(rev), np.nonzero(valid)[0])
+    assert np.array_equal(fwd[rev], np.arange(inf.num_valid))
+    assert np.all(fwd[~valid] == -1)
+    dd = O.dist_dtype(dtype)
+    assert np.array_equal(P, pts[rev].astype(dd))
+    # every leaf box is the tight box of its bucket; parents contain children
+    L, B = inf.leaf_slots, inf.leaf_size
+    for j in range(0, inf.num_leaves, 211):
+        seg = P[j * B:(j + 1) * B]
+        assert np.array_equal(box[L - 1 + j, 0], seg.min(0))
+        assert np.array_equal(box[L - 1 + j, 1], seg.max(0))
+    inner = np.arange(L - 1)
+    for c in (1, 2):
+        ch = 2 * inner + c
+        ok = box[ch, 0] <= box[ch, 1]
+        assert np.all((box[inner, 0] <= box[ch, 0])[ok])
+        assert np.all((box[inner, 1] >= box[ch, 1])[ok])
+
+
+# ------------------------------------------------------------------- kNN ----
+
+def test_knn_self_c0_float64_bit_exact(gpu):
+    """Uniform cube workload, 100k float64 points (seed 0), k=8 self query."""
+    pts = np.random.RandomState(0).rand(100_000, 3)
+    kd = _kd()
+    tree = kd._build(pts)
+    got, cnt = _rows(kd._query(tree, None, k=8))
+    want, wcnt = O.OracleTree(pts).knn(None, 8)
+    assert np.array_equal(cnt, wcnt)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("k", [1, 2, 5, 8, 16, 17, 32, 33, 64])
+def test_knn_self_float32_k(gpu, k):
+    rng = np.random.default_rng(k)
+    pts = rng.random((30_000, 3)).astype(np.float32)
+    kd = _kd()
+    got, cnt = _rows(kd._query(kd._build(pts), None, k=k))
+    want, wcnt = O.OracleTree(pts).knn(None, k)
+    assert np.array_equal(cnt, wcnt)
+    assert np.array_equal(got[:, :k], want)
+
+
+@pytest.mark.parametrize("leaf", [1, 4, 10, 16, 32, 64])
+def test_knn_leaf_size_does_not_change_results(gpu, leaf):
+    rng = np.random.default_rng(11)
+    pts = rng.normal(size=(20_000, 3))
+    kd = _kd()
+    got, _ = _rows(kd._query(kd._build(pts, maxleafsize=leaf), None, k=12))
+    want, _ = O.OracleTree(pts).knn(None, 12)
+    assert np.array_equal(got, want)
+
+
+def test_knn_dmax_and_points_queries(gpu):
+    rng = np.random.default_rng(3)
+    pts = rng.random((40_000, 3))
+    q = rng.random((5000, 3))
+    kd = _kd()
+    tree = kd._build(pts)
+    ot = O.OracleTree(pts)
+    for dmax in [0.01, 0.02, 0.05]:
+        assert_knn_equal(kd._query(tree, q, k=10, dmax=dmax),
+                         *ot.knn(q, 10, dmax=float(np.float32(dmax))))
+    # float32 queries against a float64 tree
+    assert_knn_equal(kd._query(tree, q.astype(np.float32), k=6),
+                     *ot.knn(q.astype(np.float32).astype(np.float64), 6))
+
+
+def test_knn_index_queries(gpu):
+    rng = np.random.default_rng(4)
+    n = 20_000
+    pts = rng.random((n, 3)).astype(np.float32)
+    kd = _kd()
+    tree = kd._build(pts)
+    ot = O.OracleTree(pts)
+    idx = rng.integers(-n, n, 3000)
+    got, _ = _rows(kd._query(tree, list(idx), k=7))
+    want, _ = ot.knn(np.where(idx < 0, idx + n, idx), 7)
+    assert np.array_equal(got, want)
+    mask = rng.random(n) < 0.1
+    got, _ = _rows(kd._query(tree, mask, k=7))
+    want, _ = ot.knn(np.nonzero(mask)[0], 7)
+    assert np.array_equal(got, want)
+    got, _ = _rows(kd._query(tree, np.int64(5), k=3))
+    assert np.array_equal(got, ot.knn(np.array([5]), 3)[0])
+
+
+def test_knn_ties_duplicates_grid(gpu):
+    """Integer grid with duplicated points: many exact distance ties."""
+    g = np.stack(np.meshgrid(np.arange(20), np.arange(20), np.arange(10)), -1).reshape(-1, 3)
+    pts = np.concatenate([g, g[::3], g[::7]]).astype(np.float64)
+    kd = _kd()
+    tree = kd._build(pts)
+    ot = O.OracleTree(pts)
+    for k in [1, 6, 7, 26, 27, 40]:
+        got, cnt = _rows(kd._query(tree, None, k=k))
+        want, wcnt = ot.knn(None, k)
+        assert np.array_equal(cnt, wcnt), k
+        assert np.array_equal(got[:, :k], want), k
+    bw, _ = O.brute_knn(pts, pts[:500], 9, self_rows=np.arange(500))
+    got, _ = _rows(kd._query(tree, np.arange(500), k=9))
+    assert np.array_equal(got, bw)
+
+
+@pytest.mark.parametrize("dtype", [np.int8, np.int32, np.uint16, np.int64, np.uint64])
+def test_knn_integer_dtypes(gpu, dtype):
+    rng = np.random.default_rng(8)
+    hi = 100 if dtype == np.int8 else 5000
+    pts = rng.integers(0, hi, (8000, 3)).astype(dtype)
+    kd = _kd()
+    got, cnt = _rows(kd._query(kd._build(pts), None, k=5))
+    want, wcnt = O.OracleTree(pts).knn(None, 5)
+    assert np.array_equal(cnt, wcnt)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("dim", [2, 4])
+def test_knn_dims(gpu, dim):
+    rng = np.random.default_rng(dim)
+    pts = rng.random((25_000, dim))
+    kd = _kd()
+    tree = kd._build(pts)
+    ot = O.OracleTree(pts)
+    got, _ = _rows(kd._query(tree, None, k=9))
+    assert np.array_equal(got, ot.knn(None, 9)[0])
+    q = rng.random((2000, dim)).astype(np.float32)
+    assert_knn_equal(kd._query(tree, q, k=4, dmax=0.1),
+                     *ot.knn(q.astype(np.float64), 4, dmax=float(np.float32(0.1))))
+
+
+def test_knn_nan_rows(gpu):
+    rng = np.random.default_rng(9)
+    pts = rng.random((10_000, 3))
+    pts[rng.integers(0, 10_000, 300), rng.integers(0, 3, 300)] = np.nan
+    pts[5, 1] = -np.inf
+    kd = _kd()
+    tree = kd._build(pts)
+    res = kd._query(tree, None, k=4)
+    ot = O.OracleTree(pts)
+    assert len(res) == ot.num_valid
+    got, cnt = _rows(res)
+    want, wcnt = ot.knn(None, 4)
+    assert np.array_equal(cnt, wcnt)
+    assert np.array_equal(got, want)
+
+
+def test_small_and_empty(gpu):
+    kd = _kd()
+    pts = np.random.default_rng(1).random((5, 3))
+    got = kd._query(kd._build(pts), None, k=10)
+    assert [len(x) for x in got] == [4] * 5
+    want, _ = O.OracleTree(pts).knn(None, 10)
+    assert np.array_equal(_rows(got)[0], want[:, :4])
+    empty = np.zeros((0, 3))
+    assert kd._query(kd._build(empty), None, k=3) == []
+    allnan = np.full((4, 3), np.nan)
+    assert kd._query(kd._build(allnan), np.zeros((2, 3)), k=3)[0].size == 0
+
+
+def test_deterministic(gpu):
+    rng = np.random.default_rng(12)
+    pts = rng.random((200_000, 3)).astype(np.float32)
+    kd = _kd()
+    a = np.vstack(kd._query(kd._build(pts), None, k=8))
+    b = np.vstack(kd._query(kd._build(pts), None, k=8))
+    assert np.array_equal(a, b)
+
+
+# ---------------------------------------------------------------- r-near ----
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_rnear_self_and_points(gpu, dtype):
+    rng = np.random.default_rng(21)
+    pts = rng.random((30_000, 3)).astype(dtype)
+    kd = _kd()
+    tree = kd._build(pts)
+    ot = O.OracleTree(pts)
+    for dmax in [0.02, 0.05]:
+        r = float(np.float32(dmax))
+        res = kd._query(tree, None, k=None, dmax=dmax)
+        off, idx = ot.rnear(None, r)
+        assert len(res) == len(off) - 1
+        for i in range(0, len(res), 7):
+            assert np.array_equal(np.sort(res[i]), idx[off[i]:off[i + 1]]), i
+        q = rng.random((3000, 3))
+        res = kd._query(tree, q, k=None, dmax=dmax)
+        off, idx = ot.rnear(q.astype(ot.dtype), r)
+        for i in range(len(res)):
+            assert np.array_equal(np.sort(res[i]), idx[off[i]:off[i + 1]]), i
+
+
+def test_rnear_large_radius_whole_subtrees(gpu):
+    rng = np.random.default_rng(22)
+    pts = rng.random((5000, 3)).astype(np.float32)
+    kd = _kd()
+    res = kd._query(kd._build(pts), np.arange(0, 5000, 50), k=None, dmax=0.6)
+    off, idx = O.OracleTree(pts).rnear(np.arange(0, 5000, 50), float(np.float32(0.6)))
+    for i in range(len(res)):
+        assert np.array_equal(np.sort(res[i]), idx[off[i]:off[i + 1]])
+
+
+# --------------------------------------------------------------- normals ----
+
+def _collinear():
+    y = np.zeros((10, 3), dtype=np.float64)
+    y[:, 0] = np.arange(10)
+    return y
+
+
+def test_normals_reference_output_types(gpu):
+    """pptk tests/test_estimate_normals.py:38-138 (closed form, 8 combos)."""
+    import pptk_amd
+    y = _collinear()
+    evals = np.array([2. / 3] + [5. / 4] * 8 + [2. / 3])
+    evecs = np.array([[1., 0., 0.]] * 10)
+    sizes = np.array([3] + [4] * 8 + [3])
+    for ev in (0, 1):
+        for ae in (0, 1):
+            for ns in (0, 1):
+                z = pptk_amd.estimate_normals(y, 4, 3, output_eigenvalues=ev,
+                                              output_all_eigenvectors=ae,
+                                              output_neighborhood_sizes=ns, verbose=False)
+                if not ev and not ns:
+                    assert isinstance(z, np.ndarray)
+                    assert z.shape == ((10, 3, 3) if ae else (10, 3))
+                    continue
+                assert isinstance(z, tuple) and len(z) == 3
+                assert z[0].shape == ((10, 3, 3) if ae else (10, 3))
+                assert (z[1] is None) == (not ev)
+                assert (z[2] is None) == (not ns)
+                if ns:
+                    assert np.all(z[2] == sizes)
+                if ae:
+                    assert np.allclose(z[0][:, -1, :], evecs)
+                    if ev:
+                        assert np.allclose(z[1][:, -1], evals)
+
+
+def test_normals_reference_subsample(gpu):
+    """pptk tests/test_estimate_normals.py:143-183."""
+    import pptk_amd
+    y = _collinear()
+    evals = np.array([2. / 3] + [5. / 4] * 8 + [2. / 3])
+    evecs = np.array([[1., 0., 0.]] * 10)
+    sizes = np.array([3] + [4] * 8 + [3])
+    s = [0, 3, 4, -1]
+    mask = np.zeros(10, dtype=bool)
+    mask[s] = True
+    kw = dict(output_eigenvalues=True, output_all_eigenvectors=True,
+              output_neighborhood_sizes=True, verbose=False)
+    z = pptk_amd.estimate_normals(y, 4, 3, subsample=mask, **kw)
+    sm = np.nonzero(mask)[0]
+    assert np.allclose(z[0][:, -1, :], evecs[sm])
+    assert np.allclose(z[1][:, -1], evals[sm])
+    assert np.all(z[2] == sizes[sm])
+    with pytest.raises(ValueError):
+        pptk_amd.estimate_normals(y, 4, 3, subsample=np.zeros(2, dtype=bool), **kw)
+    z = pptk_amd.estimate_normals(y, 4, 3, subsample=s, **kw)
+    assert np.allclose(z[0][:, -1, :], evecs[s])
+    assert np.allclose(z[1][:, -1], evals[s])
+    assert np.all(z[2] == sizes[s])
+
+
+@pytest.mark.parametrize("dtype,prec", [(np.float32, None), (np.float32, "fp64"),
+                                        (np.float64, None), (np.float64, "mixed")])
+def test_normals_vs_fp64_oracle(gpu, dtype, prec):
+    import pptk_amd
+    rng = np.random.default_rng(31)
+    # noisy planes: well-defined normals
+    n = 40_000
+    uv = rng.random((n, 2)) * 10
+    pts = np.stack([uv[:, 0], uv[:, 1], 0.3 * uv[:, 0] + rng.normal(0, 0.01, n)], 1)
+    pts = pts.astype(dtype)
+    kw = {} if prec is None else {"precision": prec}
+    ev, ew, sz = pptk_amd.estimate_normals(pts, 16, np.inf, output_eigenvalues=True,
+                                           output_all_eigenvectors=True,
+                                           output_neighborhood_sizes=True, verbose=False, **kw)
+    ot = O.OracleTree(pts)
+    idx, cnt = ot.knn(pts, 16)   # neighbourhood includes the point itself
+    assert np.all(sz == cnt)
+    fev, few = O.normals_fp64(pts, idx, cnt)
+    ang = O.angle_deg(ev[:, 0], fev[:, 0])
+    assert ang.max() < 0.5, ang.max()
+    assert np.allclose(ew, few, rtol=1e-3, atol=1e-6 * np.abs(few).max())
+
+
+def test_normals_radius_and_k_minus_one(gpu):
+    import pptk_amd
+    rng = np.random.default_rng(32)
+    pts = rng.random((20_000, 3))
+    ot = O.OracleTree(pts)
+    # k with a finite radius
+    ev, ew, sz = pptk_amd.estimate_normals(pts, 10, 0.03, output_eigenvalues=True,
+                                           output_all_eigenvectors=True,
+                                           output_neighborhood_sizes=True, verbose=False)
+    oev, oew, osz = ot.normals(10, float(np.float32(0.03)))
+    assert np.array_equal(sz, osz)
+    ok = sz >= 3
+    assert np.allclose(ew[ok], oew[ok], rtol=1e-6, atol=1e-12)
+    # pure r-near neighbourhoods
+    ev, ew, sz = pptk_amd.estimate_normals(pts, -1, 0.05, output_eigenvalues=True,
+                                           output_all_eigenvectors=True,
+                                           output_neighborhood_sizes=True, verbose=False)
+    oev, oew, osz = ot.normals(-1, float(np.float32(0.05)))
+    assert np.array_equal(sz, osz)
+    ok = sz >= 3
+    assert np.allclose(ew[ok], oew[ok], rtol=1e-6, atol=1e-12)
+    # k < -1: empty neighbourhoods, NaN output (estimate_normals.cpp:79-82)
+    z = pptk_amd.estimate_normals(pts[:100], -2, 0.05, output_neighborhood_sizes=True,
+                                  verbose=False)
+    assert np.all(z[2] == 0) and np.all(np.isnan(z[0]))
+
+
+def test_normals_large_k(gpu):
+    import pptk_amd
+    rng = np.random.default_rng(33)
+    pts = rng.random((5000, 3))
+    ev, ew, sz = pptk_amd.estimate_normals(pts, 50, np.inf, output_eigenvalues=True,
+                                           output_all_eigenvectors=True,
+                                           output_neighborhood_sizes=True, verbose=False)
+    oev, oew, osz = O.OracleTree(pts).normals(50, np.inf)
+    assert np.array_equal(sz, osz)
+    assert np.allclose(ew, oew, rtol=1e-9, atol=1e-14)

@@ -392,7 +392,9 @@ void build_impl(pptk_tree* t, const T* dpts, int64_t n, hipStream_t s) {
     t->hi[d] = d < DIM ? (double)info.hi[d] : 0.0;
   }
 
-  // Bucket geometry.
+  // Bucket geometry. Node ids are int32, so keep leaf slots <= 2^30.
+  while (t->leaf < 64 && ceil_div(nv, t->leaf) > ((int64_t)1 << 30)) t->leaf *= 2;
+  if (t->leaf > 64) t->leaf = 64;
   const int64_t B = t->leaf;
   t->nleaves = ceil_div(nv, B);
   int64_t L = 1;

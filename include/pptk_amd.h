@@ -105,10 +105,10 @@ typedef struct {
   pptk_dtype dist_dtype; /* PPTK_F32 or PPTK_F64 */
   int64_t num_input;     /* rows passed to build */
   int64_t num_valid;     /* rows without NaN/inf */
-  int leaf_size;
+  int leaf_size;         /* maximum points per leaf */
   int64_t num_leaves;
-  int64_t leaf_slots;    /* power of two >= num_leaves */
-  int depth;             /* log2(leaf_slots) */
+  int64_t num_nodes;
+  int depth;             /* number of node levels */
   double bbox_lo[4];
   double bbox_hi[4];
   int64_t device_bytes;  /* device memory held by the tree */
@@ -145,11 +145,12 @@ pptk_status pptk_tree_get_info(const pptk_tree* tree, pptk_tree_info* out);
  *   points  : num_valid x dim in tree order, dist dtype
  *   reverse : num_valid int32, tree position -> original index
  *   forward : num_input int32, original index -> tree position or -1
- *   boxes   : (2*leaf_slots-1) x 2 x dim dist dtype; node i = [lo(dim), hi(dim)],
- *             children of i are 2i+1 and 2i+2, leaf j is node leaf_slots-1+j
- *             and holds tree positions [j*leaf_size, min(num_valid,(j+1)*leaf_size)). */
+ *   boxes   : num_nodes x 2 x dim dist dtype, node i = [lo(dim), hi(dim)]
+ *   nodes   : num_nodes x 4 int32, node i = [a, b, child, parent]: tree
+ *             positions [a, b), first child (children are child, child+1)
+ *             or -1 for a leaf, parent or -1 for the root (node 0). */
 pptk_status pptk_tree_export(const pptk_tree* tree, void* points, int32_t* reverse,
-                             int32_t* forward, void* boxes, pptk_mem mem);
+                             int32_t* forward, void* boxes, int32_t* nodes, pptk_mem mem);
 
 /* k nearest neighbours with distance < dmax. Replaces
  * KdTree::KNearestNeighbors / KNearestNeighborsSelf batch calls

@@ -53,7 +53,7 @@ class TreeInfo(ctypes.Structure):
     _fields_ = [("dim", ctypes.c_int), ("dtype", ctypes.c_int), ("dist_dtype", ctypes.c_int),
                 ("num_input", ctypes.c_int64), ("num_valid", ctypes.c_int64),
                 ("leaf_size", ctypes.c_int), ("num_leaves", ctypes.c_int64),
-                ("leaf_slots", ctypes.c_int64), ("depth", ctypes.c_int),
+                ("num_nodes", ctypes.c_int64), ("depth", ctypes.c_int),
                 ("bbox_lo", ctypes.c_double * 4), ("bbox_hi", ctypes.c_double * 4),
                 ("device_bytes", ctypes.c_int64)]
 
@@ -80,7 +80,7 @@ def lib():
     L.pptk_tree_get_info.restype = ci
     L.pptk_tree_get_info.argtypes = [vp, ctypes.POINTER(TreeInfo)]
     L.pptk_tree_export.restype = ci
-    L.pptk_tree_export.argtypes = [vp, vp, vp, vp, vp, ci]
+    L.pptk_tree_export.argtypes = [vp, vp, vp, vp, vp, vp, ci]
     L.pptk_knn.restype = ci
     L.pptk_knn.argtypes = [vp, ctypes.POINTER(Queries), ci, cd, vp, vp, vp, ci, vp]
     L.pptk_rnear_count.restype = ci
@@ -183,17 +183,18 @@ class Tree:
         return np.float64 if self.info.dist_dtype == F64 else np.float32
 
     def export(self):
-        """Host copies: points (tree order), reverse, forward, boxes (nodes, 2, dim)."""
+        """Host copies: points (tree order), reverse, forward, boxes (nn, 2, dim),
+        nodes (nn, 4) = [a, b, child, parent]."""
         inf = self.info
         dt = self.dist_dtype
         pts = np.empty((inf.num_valid, inf.dim), dt)
         rev = np.empty(inf.num_valid, np.int32)
         fwd = np.empty(inf.num_input, np.int32)
-        nodes = 2 * inf.leaf_slots - 1
-        box = np.empty((nodes, 2, inf.dim), dt)
+        box = np.empty((inf.num_nodes, 2, inf.dim), dt)
+        nodes = np.empty((inf.num_nodes, 4), np.int32)
         check(lib().pptk_tree_export(ctypes.c_void_p(self.handle), _p(pts), _p(rev), _p(fwd),
-                                     _p(box), MEM_HOST))
-        return pts, rev, fwd, box
+                                     _p(box), _p(nodes), MEM_HOST))
+        return pts, rev, fwd, box, nodes
 
 
 def make_queries(queries=None, *, dev_ptr=None, m=None, kind=None, dtype=None, mem=MEM_HOST):

@@ -12,13 +12,20 @@
 //   K2 radix sort      (sort.hip) on the 63-bit code
 //   K3 gather          tree-order point records, reverse/forward maps
 //                      (kdtree-impl.h:803-817)
-//   K3 leaf/level boxes buckets of `leaf` consecutive points; implicit
-//                      complete binary tree (children 2i+1, 2i+2) whose node
-//                      boxes are the tight union of their buckets, built
-//                      level by level from the leaves up.
+//   K3 split levels    top-down, one launch per level: a node holding more
+//                      than `leaf` points splits at the highest Morton bit in
+//                      which its first and last codes differ (binary search
+//                      for the first code with that bit set), or at the middle
+//                      when all its codes are equal. Every node is a Morton
+//                      prefix cell, siblings are disjoint cells, and a leaf is
+//                      a cell of at most `leaf` points. Children are allocated
+//                      in adjacent pairs into one flat node array.
+//   K3 node boxes      bottom-up, one launch per level: tight box of each
+//                      leaf's points, union of the children's boxes above.
 #include <algorithm>
 #include <mutex>
 #include <set>
+#include <vector>
 
 #include "sort.hpp"
 #include "tree.hpp"
@@ -199,9 +206,16 @@ __global__ __launch_bounds__(kBlock) void ingest_finalize_kernel(const uint32_t*
       }
       info->lo[d] = l;
       info->hi[d] = h;
-      const D wdt = h - l;
-      info->scale[d] = (d < DIM && wdt > D(0) && finite_val(wdt)) ? maxq / wdt : D(0);
     }
+    // One scale for every axis: cubic Morton cells keep node boxes compact
+    // on flat or elongated clouds.
+    D ext = D(0);
+    for (int d = 0; d < DIM; ++d) {
+      const D wdt = info->hi[d] - info->lo[d];
+      ext = wdt > ext ? wdt : ext;
+    }
+    for (int d = 0; d < 4; ++d)
+      info->scale[d] = (d < DIM && ext > D(0) && finite_val(ext)) ? maxq / ext : D(0);
   }
 }
 
@@ -314,22 +328,61 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(const T* __restrict__ pt
   fwd[i] = (int32_t)p;
 }
 
+// One level of the top-down split. Nodes [s, e) are this level; a node with
+// more than `leaf` points gets two children appended at *counter.
+__global__ __launch_bounds__(kBlock) void split_level_kernel(const uint64_t* __restrict__ keys,
+                                                             NodeRec* __restrict__ nodes,
+                                                             int64_t s, int64_t e, int leaf,
+                                                             int* __restrict__ counter) {
+  const int64_t i = s + (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= e) return;
+  const int32_t a = nodes[i].a, b = nodes[i].b;
+  if (b - a <= leaf) {
+    nodes[i].child = -1;
+    return;
+  }
+  const uint64_t x = keys[a] ^ keys[b - 1];
+  int32_t split;
+  if (x == 0ull) {
+    split = a + (b - a) / 2;
+  } else {
+    // Codes in [a, b) share every bit above `bit`; keys[a] has it clear and
+    // keys[b-1] set, so the first code with it set lies in (a, b-1].
+    const int bit = 63 - __clzll((long long)x);
+    int32_t lo = a + 1, hi = b - 1;
+    while (lo < hi) {
+      const int32_t mid = lo + ((hi - lo) >> 1);
+      if ((keys[mid] >> bit) & 1ull) hi = mid;
+      else lo = mid + 1;
+    }
+    split = lo;
+  }
+  const int c = atomicAdd(counter, 2);
+  NodeRec l, r;
+  l.a = a; l.b = split; l.child = -1; l.parent = (int32_t)i;
+  r.a = split; r.b = b; r.child = -1; r.parent = (int32_t)i;
+  nodes[c] = l;
+  nodes[c + 1] = r;
+  nodes[i].child = c;
+}
+
+// One level of the bottom-up box pass over nodes [s, e).
 template <typename D, int DIM>
-__global__ __launch_bounds__(kBlock) void leaf_box_kernel(const PointRec<D, DIM>* __restrict__ rec,
-                                                          int64_t nv, int leaf, int64_t nleaves,
-                                                          int64_t L, BoxRec<D, DIM>* __restrict__ box) {
-  const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (j >= L) return;
+__global__ __launch_bounds__(kBlock) void node_box_kernel(const PointRec<D, DIM>* __restrict__ rec,
+                                                          const NodeRec* __restrict__ nodes,
+                                                          BoxRec<D, DIM>* __restrict__ box,
+                                                          int64_t s, int64_t e) {
+  const int64_t i = s + (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= e) return;
+  const NodeRec nr = nodes[i];
   BoxRec<D, DIM> b;
 #pragma unroll
   for (int d = 0; d < PadDim<DIM>::value; ++d) {
     b.lo[d] = d < DIM ? pos_inf<D>() : D(0);
     b.hi[d] = d < DIM ? -pos_inf<D>() : D(0);
   }
-  if (j < nleaves) {
-    const int64_t s = j * leaf;
-    const int64_t e = s + leaf < nv ? s + leaf : nv;
-    for (int64_t p = s; p < e; ++p) {
+  if (nr.child < 0) {
+    for (int32_t p = nr.a; p < nr.b; ++p) {
       const PointRec<D, DIM> r = rec[p];
 #pragma unroll
       for (int d = 0; d < DIM; ++d) {
@@ -337,23 +390,14 @@ __global__ __launch_bounds__(kBlock) void leaf_box_kernel(const PointRec<D, DIM>
         b.hi[d] = r.c[d] > b.hi[d] ? r.c[d] : b.hi[d];
       }
     }
-  }
-  box[L - 1 + j] = b;
-}
-
-template <typename D, int DIM>
-__global__ __launch_bounds__(kBlock) void level_box_kernel(BoxRec<D, DIM>* __restrict__ box,
-                                                           int64_t first, int64_t count) {
-  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (t >= count) return;
-  const int64_t i = first + t;
-  const BoxRec<D, DIM> a = box[2 * i + 1];
-  const BoxRec<D, DIM> c = box[2 * i + 2];
-  BoxRec<D, DIM> b;
+  } else {
+    const BoxRec<D, DIM> x = box[nr.child];
+    const BoxRec<D, DIM> y = box[nr.child + 1];
 #pragma unroll
-  for (int d = 0; d < PadDim<DIM>::value; ++d) {
-    b.lo[d] = a.lo[d] < c.lo[d] ? a.lo[d] : c.lo[d];
-    b.hi[d] = a.hi[d] > c.hi[d] ? a.hi[d] : c.hi[d];
+    for (int d = 0; d < DIM; ++d) {
+      b.lo[d] = x.lo[d] < y.lo[d] ? x.lo[d] : y.lo[d];
+      b.hi[d] = x.hi[d] > y.hi[d] ? x.hi[d] : y.hi[d];
+    }
   }
   box[i] = b;
 }
@@ -392,27 +436,29 @@ void build_impl(pptk_tree* t, const T* dpts, int64_t n, hipStream_t s) {
     t->hi[d] = d < DIM ? (double)info.hi[d] : 0.0;
   }
 
-  // Bucket geometry. Node ids are int32, so keep leaf slots <= 2^30.
-  while (t->leaf < 64 && ceil_div(nv, t->leaf) > ((int64_t)1 << 30)) t->leaf *= 2;
-  if (t->leaf > 64) t->leaf = 64;
   const int64_t B = t->leaf;
-  t->nleaves = ceil_div(nv, B);
-  int64_t L = 1;
-  int depth = 0;
-  while (L < t->nleaves) { L <<= 1; ++depth; }
-  t->L = L;
-  t->depth = depth;
-
   t->pts.reset((size_t)std::max<int64_t>(nv, 1) * sizeof(PointRec<D, DIM>));
   t->orig.reset((size_t)std::max<int64_t>(nv, 1) * 4);
   t->fwd.reset((size_t)std::max<int64_t>(n, 1) * 4);
-  t->box.reset((size_t)(2 * L - 1) * sizeof(BoxRec<D, DIM>));
   if (n > 0) PK_HIP(hipMemsetAsync(t->fwd.get(), 0xff, (size_t)n * 4, s));
 
+  // A binary tree whose leaves are non-empty has at most 2*nv - 1 nodes.
+  const int64_t cap = std::max<int64_t>(1, 2 * nv - 1);
+  DevBuf nodes((size_t)cap * sizeof(NodeRec));
+  DevBuf counter(sizeof(int));
+  std::vector<std::pair<int64_t, int64_t>> levels;
+  {
+    NodeRec root;
+    root.a = 0; root.b = (int32_t)nv; root.child = -1; root.parent = -1;
+    const int one = 1;
+    PK_HIP(hipMemcpyAsync(nodes.get(), &root, sizeof(root), hipMemcpyHostToDevice, s));
+    PK_HIP(hipMemcpyAsync(counter.get(), &one, sizeof(int), hipMemcpyHostToDevice, s));
+  }
+  DevBuf keys, vals;
   if (nv > 0) {
     // K1: compaction + Morton codes
-    DevBuf keys((size_t)nv * 8);
-    DevBuf vals((size_t)nv * 4);
+    keys.reset((size_t)nv * 8);
+    vals.reset((size_t)nv * 4);
     morton_kernel<T, D, DIM><<<(unsigned)nblk, kBlock, 0, s>>>(
         dpts, n, blk_off.as<uint32_t>(), info_d.as<BuildInfo<D>>(), keys.as<uint64_t>(),
         vals.as<uint32_t>());
@@ -424,19 +470,39 @@ void build_impl(pptk_tree* t, const T* dpts, int64_t n, hipStream_t s) {
         dpts, vals.as<uint32_t>(), nv, t->pts.as<PointRec<D, DIM>>(), t->orig.as<int32_t>(),
         t->fwd.as<int32_t>());
     PK_LAUNCH_CHECK();
-    PK_HIP(hipStreamSynchronize(s));  // keys/vals return to the pool
   }
-  leaf_box_kernel<D, DIM><<<grid_for(L, kBlock), kBlock, 0, s>>>(
-      t->pts.as<PointRec<D, DIM>>(), nv, (int)B, t->nleaves, L, t->box.as<BoxRec<D, DIM>>());
-  PK_LAUNCH_CHECK();
-  for (int lvl = depth - 1; lvl >= 0; --lvl) {
-    const int64_t first = ((int64_t)1 << lvl) - 1;
-    const int64_t count = (int64_t)1 << lvl;
-    level_box_kernel<D, DIM><<<grid_for(count, kBlock), kBlock, 0, s>>>(
-        t->box.as<BoxRec<D, DIM>>(), first, count);
+  // K3: split level by level until a level creates no children.
+  int64_t ls = 0, le = 1;
+  while (ls < le) {
+    levels.emplace_back(ls, le);
+    if (nv > 0) {
+      split_level_kernel<<<grid_for(le - ls, kBlock), kBlock, 0, s>>>(
+          keys.as<uint64_t>(), nodes.as<NodeRec>(), ls, le, (int)B, counter.as<int>());
+      PK_LAUNCH_CHECK();
+    }
+    int cnt = 0;
+    PK_HIP(hipMemcpyAsync(&cnt, counter.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+    PK_HIP(hipStreamSynchronize(s));
+    if (cnt > cap) fail(PPTK_E_RUNTIME, "tree build: node count %d exceeds bound %lld", cnt,
+                        (long long)cap);
+    ls = le;
+    le = cnt;
+  }
+  const int64_t nn = le;
+  t->nn = nn;
+  t->depth = (int)levels.size();
+  t->nleaves = (nn + 1) / 2;
+  t->node.reset((size_t)nn * sizeof(NodeRec));
+  PK_HIP(hipMemcpyAsync(t->node.get(), nodes.get(), (size_t)nn * sizeof(NodeRec),
+                        hipMemcpyDeviceToDevice, s));
+  t->box.reset((size_t)nn * sizeof(BoxRec<D, DIM>));
+  for (auto it = levels.rbegin(); it != levels.rend(); ++it) {
+    node_box_kernel<D, DIM><<<grid_for(it->second - it->first, kBlock), kBlock, 0, s>>>(
+        t->pts.as<PointRec<D, DIM>>(), t->node.as<NodeRec>(), t->box.as<BoxRec<D, DIM>>(),
+        it->first, it->second);
     PK_LAUNCH_CHECK();
   }
-  PK_HIP(hipStreamSynchronize(s));
+  PK_HIP(hipStreamSynchronize(s));  // temporaries return to the pool
 }
 
 std::mutex g_trees_mu;
@@ -469,7 +535,7 @@ pptk_status api_tree_build(const void* pts, int64_t n, int dim, pptk_dtype dtype
     if (dim < 2 || dim > 4) pk::fail(PPTK_E_RUNTIME, "Invalid k-d tree dim = %d.", dim);
     if (!pk::dtype_valid(dtype)) pk::fail(PPTK_E_RUNTIME, "Invalid k-d tree type_num = %d.", (int)dtype);
     if (n < 0) pk::fail(PPTK_E_VALUE, "number of points must be non-negative");
-    if (n > 0x7fffffffLL) pk::fail(PPTK_E_VALUE, "at most 2^31-1 points per tree (int32 indices)");
+    if (n > (1LL << 30)) pk::fail(PPTK_E_VALUE, "at most 2^30 points per tree (int32 node ids)");
     if (n > 0 && !pts) pk::fail(PPTK_E_VALUE, "null points");
     hipStream_t s = pk::pick_stream(stream);
     std::unique_ptr<pptk_tree> t(new pptk_tree());
@@ -543,11 +609,11 @@ extern "C" pptk_status pptk_tree_get_info(const pptk_tree* tree, pptk_tree_info*
     out->num_valid = tree->n_valid;
     out->leaf_size = tree->leaf;
     out->num_leaves = tree->nleaves;
-    out->leaf_slots = tree->L;
+    out->num_nodes = tree->nn;
     out->depth = tree->depth;
     for (int d = 0; d < 4; ++d) { out->bbox_lo[d] = tree->lo[d]; out->bbox_hi[d] = tree->hi[d]; }
     out->device_bytes = (int64_t)(tree->pts.bytes() + tree->orig.bytes() + tree->fwd.bytes() +
-                                  tree->box.bytes());
+                                  tree->box.bytes() + tree->node.bytes());
   });
 }
 
@@ -579,12 +645,13 @@ __global__ __launch_bounds__(256) void export_boxes_kernel(const BoxRec<D, DIM>*
 
 namespace pk {
 pptk_status api_tree_export(const pptk_tree* tree, void* points, int32_t* reverse,
-                                        int32_t* forward, void* boxes, pptk_mem mem) {
+                                        int32_t* forward, void* boxes, int32_t* nodes,
+                                        pptk_mem mem) {
   return pk::guard([&] {
     pk::check_tree(tree);
     hipStream_t s = pk::lib_stream();
     const int64_t nv = tree->n_valid;
-    const int64_t nn = 2 * tree->L - 1;
+    const int64_t nn = tree->nn;
     const pptk_tree* t = tree;
     auto run = [&](auto dt, auto dimc) {
         using D = typename decltype(dt)::type;
@@ -613,12 +680,14 @@ pptk_status api_tree_export(const pptk_tree* tree, void* points, int32_t* revers
     if (reverse && nv > 0) PK_HIP(hipMemcpyAsync(reverse, tree->orig.get(), (size_t)nv * 4, k, s));
     if (forward && tree->n_input > 0)
       PK_HIP(hipMemcpyAsync(forward, tree->fwd.get(), (size_t)tree->n_input * 4, k, s));
+    if (nodes) PK_HIP(hipMemcpyAsync(nodes, tree->node.get(), (size_t)nn * 16, k, s));
     PK_HIP(hipStreamSynchronize(s));
   });
 }
 }  // namespace pk
 
 extern "C" pptk_status pptk_tree_export(const pptk_tree* tree, void* points, int32_t* reverse,
-                                        int32_t* forward, void* boxes, pptk_mem mem) {
-  return pk::api_tree_export(tree, points, reverse, forward, boxes, mem);
+                                        int32_t* forward, void* boxes, int32_t* nodes,
+                                        pptk_mem mem) {
+  return pk::api_tree_export(tree, points, reverse, forward, boxes, nodes, mem);
 }

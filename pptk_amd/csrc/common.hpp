@@ -222,17 +222,25 @@ template <typename D> __device__ __forceinline__ bool finite_val(D x) {
   return x - x == D(0);  // false for NaN and +-inf
 }
 
+// Tree node: points [a, b) in tree order; `child` is the first of the two
+// adjacent children (child, child + 1), or -1 for a leaf; `parent` is -1 for
+// the root. Node 0 is the root; children are allocated in pairs at odd ids,
+// so a node's sibling is id + 1 (odd id) or id - 1 (even id).
+struct NodeRec {
+  int32_t a, b, child, parent;
+};
+
 // Device view of a built tree, passed by value to the query kernels.
 template <typename D, int DIM>
 struct TreeView {
   const PointRec<D, DIM>* pts;  // num_valid records, tree order
   const int32_t* orig;          // num_valid, tree position -> original index
   const int32_t* fwd;           // num_input, original index -> position or -1
-  const BoxRec<D, DIM>* box;    // 2*L-1 nodes, heap order
+  const BoxRec<D, DIM>* box;    // nn node boxes
+  const NodeRec* node;          // nn nodes
   int32_t n;                    // num_valid
-  int32_t leaf;                 // bucket size B
-  int32_t L;                    // leaf slots (power of two)
-  int32_t depth;                // log2(L)
+  int32_t leaf;                 // maximum points per leaf
+  int32_t nn;                   // number of nodes
   int64_t n_input;
 };
 

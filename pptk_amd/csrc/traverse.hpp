@@ -1,10 +1,12 @@
 // Per-lane tree traversal shared by the kNN, r-near and normals kernels.
 //
-// The tree is an implicit complete binary tree (node i has children 2i+1 and
-// 2i+2, leaf j is node L-1+j and holds tree positions [j*B, (j+1)*B)), so a
-// lane walks it without a stack: it keeps its node, its depth and one bit per
-// level recording which child it entered first. Backtracking reads the
-// sibling from that bit and re-tests it against the current k-th bound.
+// Nodes are Morton prefix cells holding contiguous tree positions [a, b);
+// children come in adjacent pairs (child, child + 1) and a node's sibling is
+// id + 1 for an odd id, id - 1 for an even id. A lane walks the tree without
+// a stack: it keeps its node, its depth and one bit per level recording which
+// child it entered first (two 64-bit words: depth stays below 64 key bits plus
+// 31 levels of equal-code halving). Backtracking reads the sibling from that
+// bit and re-tests it against the current k-th bound.
 //
 // Neighbour order is the key (d, original index): smaller squared distance
 // first, ties broken by the smaller original index. A node is entered while
@@ -129,40 +131,50 @@ struct HeapK {
 };
 
 template <typename D, int DIM, typename State>
-__device__ __forceinline__ void knn_scan_leaf(const TreeView<D, DIM>& tv, int32_t j, const D* q,
-                                              D r2, int32_t excl, State& st) {
-  const int32_t base = j * tv.leaf;
-  const int32_t rem = tv.n - base;
-  const int32_t cnt = rem < tv.leaf ? rem : tv.leaf;
+__device__ __forceinline__ void knn_scan_leaf(const TreeView<D, DIM>& tv, int32_t a, int32_t b,
+                                              const D* q, D r2, int32_t excl, State& st) {
   const D kb = st.bound();
   uint64_t mask = 0;
-  for (int32_t s = 0; s < cnt; ++s) {
-    const PointRec<D, DIM> r = tv.pts[base + s];
+  for (int32_t p = a; p < b; ++p) {
+    const PointRec<D, DIM> r = tv.pts[p];
     const D dd = dist2<D, DIM>(q, r.c);
-    const bool pass = (dd < r2) && (dd <= kb) && (base + s != excl);
-    mask |= (uint64_t)pass << s;
+    const bool pass = (dd < r2) && (dd <= kb) && (p != excl);
+    mask |= (uint64_t)pass << (p - a);
   }
   while (mask) {
     const int s = __builtin_ctzll(mask);
     mask &= mask - 1ull;
-    const PointRec<D, DIM> r = tv.pts[base + s];
+    const PointRec<D, DIM> r = tv.pts[a + s];
     const D dd = dist2<D, DIM>(q, r.c);
-    if (st.beats(dd, r.orig)) st.insert(dd, r.orig, base + s);
+    if (st.beats(dd, r.orig)) st.insert(dd, r.orig, a + s);
   }
+}
+
+__device__ __forceinline__ void set_flip(uint64_t& lo, uint64_t& hi, int depth, bool v) {
+  if (depth < 64) {
+    const uint64_t bit = 1ull << depth;
+    lo = v ? (lo | bit) : (lo & ~bit);
+  } else {
+    const uint64_t bit = 1ull << (depth - 64);
+    hi = v ? (hi | bit) : (hi & ~bit);
+  }
+}
+__device__ __forceinline__ bool get_flip(uint64_t lo, uint64_t hi, int depth) {
+  return depth < 64 ? ((lo >> depth) & 1ull) : ((hi >> (depth - 64)) & 1ull);
 }
 
 // k-nearest search from the root, near child first.
 template <typename D, int DIM, typename State>
 __device__ void knn_search(const TreeView<D, DIM>& tv, const D* q, D r2, int32_t excl, State& st) {
-  const int32_t leaf0 = tv.L - 1;
   if (!(box_min_dist2<D, DIM>(tv.box[0], q) < r2)) return;
   int32_t node = 0;
   int depth = 0;
-  uint32_t flip = 0;
+  uint64_t flo = 0, fhi = 0;
+  NodeRec cur = tv.node[0];
   for (;;) {
     bool at_leaf = true;
-    while (node < leaf0) {
-      const int32_t c0 = 2 * node + 1;
+    while (cur.child >= 0) {
+      const int32_t c0 = cur.child;
       const D m0 = box_min_dist2<D, DIM>(tv.box[c0], q);
       const D m1 = box_min_dist2<D, DIM>(tv.box[c0 + 1], q);
       const D kb = st.bound();
@@ -170,25 +182,37 @@ __device__ void knn_search(const TreeView<D, DIM>& tv, const D* q, D r2, int32_t
       const bool n1 = (m1 < r2) && (m1 <= kb);
       if (!(n0 || n1)) { at_leaf = false; break; }
       const bool right = n1 && (!n0 || m1 < m0);
-      const uint32_t bit = 1u << depth;
-      flip = right ? (flip | bit) : (flip & ~bit);
+      set_flip(flo, fhi, depth, right);
       node = c0 + (right ? 1 : 0);
       ++depth;
+      cur = tv.node[node];
     }
-    if (at_leaf) knn_scan_leaf<D, DIM>(tv, node - leaf0, q, r2, excl, st);
+    if (at_leaf) knn_scan_leaf<D, DIM>(tv, cur.a, cur.b, q, r2, excl, st);
     for (;;) {
       if (depth == 0) return;
       const bool is_right = (node & 1) == 0;
-      const bool first_right = (flip >> (depth - 1)) & 1u;
-      if (is_right == first_right) {
+      if (is_right == get_flip(flo, fhi, depth - 1)) {
         const int32_t sib = is_right ? node - 1 : node + 1;
         const D ms = box_min_dist2<D, DIM>(tv.box[sib], q);
-        if (ms < r2 && ms <= st.bound()) { node = sib; break; }
+        if (ms < r2 && ms <= st.bound()) {
+          node = sib;
+          cur = tv.node[sib];
+          break;
+        }
       }
-      node = (node - 1) >> 1;
+      node = cur.parent;
+      cur = tv.node[node];
       --depth;
     }
   }
+}
+
+// Next node after finishing `node`'s subtree in left-to-right order, or 0
+// when the walk is complete.
+template <typename D, int DIM>
+__device__ __forceinline__ int32_t next_in_order(const TreeView<D, DIM>& tv, int32_t node) {
+  while (node != 0 && (node & 1) == 0) node = tv.node[node].parent;
+  return node == 0 ? 0 : node + 1;
 }
 
 // r-near walk in tree order (left subtree first). Counts (FILL = false) or
@@ -199,35 +223,25 @@ template <typename D, int DIM, bool FILL>
 __device__ int64_t rnear_walk(const TreeView<D, DIM>& tv, const D* q, D r2, int32_t excl,
                               int32_t* __restrict__ out, int64_t cap) {
   int64_t cnt = 0;
-  const int32_t leaf0 = tv.L - 1;
   int32_t node = 0;
   for (;;) {
-    bool descend = false;
     const BoxRec<D, DIM> b = tv.box[node];
     if (box_min_dist2<D, DIM>(b, q) < r2) {
+      const NodeRec nr = tv.node[node];
       if (box_max_dist2<D, DIM>(b, q) < r2) {
-        const int dd = 31 - __clz(node + 1);
-        const int sh = tv.depth - dd;
-        const int64_t j0 = ((int64_t)(node + 1) << sh) - (int64_t)tv.L;
-        const int64_t s0 = j0 * tv.leaf;
-        int64_t s1 = (j0 + ((int64_t)1 << sh)) * tv.leaf;
-        if (s1 > tv.n) s1 = tv.n;
         if constexpr (FILL) {
-          for (int64_t p = s0; p < s1; ++p) {
+          for (int32_t p = nr.a; p < nr.b; ++p) {
             if (p == excl) continue;
             if (cnt < cap) out[cnt] = tv.orig[p];
             ++cnt;
           }
         } else {
-          cnt += (s1 - s0) - ((excl >= s0 && excl < s1) ? 1 : 0);
+          cnt += (nr.b - nr.a) - ((excl >= nr.a && excl < nr.b) ? 1 : 0);
         }
-      } else if (node >= leaf0) {
-        const int32_t base = (node - leaf0) * tv.leaf;
-        const int32_t rem = tv.n - base;
-        const int32_t c = rem < tv.leaf ? rem : tv.leaf;
-        for (int32_t s = 0; s < c; ++s) {
-          const PointRec<D, DIM> r = tv.pts[base + s];
-          if (dist2<D, DIM>(q, r.c) < r2 && base + s != excl) {
+      } else if (nr.child < 0) {
+        for (int32_t p = nr.a; p < nr.b; ++p) {
+          const PointRec<D, DIM> r = tv.pts[p];
+          if (dist2<D, DIM>(q, r.c) < r2 && p != excl) {
             if constexpr (FILL) {
               if (cnt < cap) out[cnt] = r.orig;
             }
@@ -235,14 +249,12 @@ __device__ int64_t rnear_walk(const TreeView<D, DIM>& tv, const D* q, D r2, int3
           }
         }
       } else {
-        node = 2 * node + 1;
-        descend = true;
+        node = nr.child;
+        continue;
       }
     }
-    if (descend) continue;
-    while (node != 0 && (node & 1) == 0) node = (node - 1) >> 1;
+    node = next_in_order<D, DIM>(tv, node);
     if (node == 0) break;
-    node = node + 1;
   }
   return cnt;
 }
@@ -251,37 +263,25 @@ __device__ int64_t rnear_walk(const TreeView<D, DIM>& tv, const D* q, D r2, int3
 // the r-near normals; the query point itself is included).
 template <typename D, int DIM, typename V>
 __device__ void rnear_visit(const TreeView<D, DIM>& tv, const D* q, D r2, V& visit) {
-  const int32_t leaf0 = tv.L - 1;
   int32_t node = 0;
   for (;;) {
-    bool descend = false;
     const BoxRec<D, DIM> b = tv.box[node];
     if (box_min_dist2<D, DIM>(b, q) < r2) {
+      const NodeRec nr = tv.node[node];
       if (box_max_dist2<D, DIM>(b, q) < r2) {
-        const int dd = 31 - __clz(node + 1);
-        const int sh = tv.depth - dd;
-        const int64_t j0 = ((int64_t)(node + 1) << sh) - (int64_t)tv.L;
-        const int64_t s0 = j0 * tv.leaf;
-        int64_t s1 = (j0 + ((int64_t)1 << sh)) * tv.leaf;
-        if (s1 > tv.n) s1 = tv.n;
-        for (int64_t p = s0; p < s1; ++p) visit(tv.pts[p]);
-      } else if (node >= leaf0) {
-        const int32_t base = (node - leaf0) * tv.leaf;
-        const int32_t rem = tv.n - base;
-        const int32_t c = rem < tv.leaf ? rem : tv.leaf;
-        for (int32_t s = 0; s < c; ++s) {
-          const PointRec<D, DIM> r = tv.pts[base + s];
+        for (int32_t p = nr.a; p < nr.b; ++p) visit(tv.pts[p]);
+      } else if (nr.child < 0) {
+        for (int32_t p = nr.a; p < nr.b; ++p) {
+          const PointRec<D, DIM> r = tv.pts[p];
           if (dist2<D, DIM>(q, r.c) < r2) visit(r);
         }
       } else {
-        node = 2 * node + 1;
-        descend = true;
+        node = nr.child;
+        continue;
       }
     }
-    if (descend) continue;
-    while (node != 0 && (node & 1) == 0) node = (node - 1) >> 1;
+    node = next_in_order<D, DIM>(tv, node);
     if (node == 0) break;
-    node = node + 1;
   }
 }
 

@@ -11,15 +11,16 @@ struct pptk_tree {
   int64_t n_input = 0;
   int64_t n_valid = 0;
   int leaf = 16;
-  int depth = 0;
+  int depth = 0;          // number of node levels
   int64_t nleaves = 0;
-  int64_t L = 1;
+  int64_t nn = 1;         // number of nodes
   double lo[4] = {0, 0, 0, 0};
   double hi[4] = {0, 0, 0, 0};
   pk::DevBuf pts;   // PointRec<D,dim>[n_valid]
   pk::DevBuf orig;  // int32[n_valid]
   pk::DevBuf fwd;   // int32[n_input]
-  pk::DevBuf box;   // BoxRec<D,dim>[2L-1]
+  pk::DevBuf box;   // BoxRec<D,dim>[nn]
+  pk::DevBuf node;  // NodeRec[nn]
 };
 
 namespace pk {
@@ -33,10 +34,10 @@ TreeView<D, DIM> tree_view(const pptk_tree* t) {
   v.orig = t->orig.as<const int32_t>();
   v.fwd = t->fwd.as<const int32_t>();
   v.box = t->box.as<const BoxRec<D, DIM>>();
+  v.node = t->node.as<const NodeRec>();
   v.n = (int32_t)t->n_valid;
   v.leaf = t->leaf;
-  v.L = (int32_t)t->L;
-  v.depth = t->depth;
+  v.nn = (int32_t)t->nn;
   v.n_input = t->n_input;
   return v;
 }

@@ -74,7 +74,7 @@ def test_tree_invariants(gpu, dim, dtype):
         pts[::89, 0] = np.inf
     t = capi.Tree.build(pts, leaf=13)
     inf = t.info
-    P, rev, fwd, box = t.export()
+    P, rev, fwd, box, nodes = t.export()
     valid = np.all(np.isfinite(pts.astype(np.float64)), axis=1)
     assert inf.num_valid == valid.sum()
     # reverse is a permutation of the valid rows; forward inverts it
@@ -83,18 +83,49 @@ def test_tree_invariants(gpu, dim, dtype):
     assert np.all(fwd[~valid] == -1)
     dd = O.dist_dtype(dtype)
     assert np.array_equal(P, pts[rev].astype(dd))
-    # every leaf box is the tight box of its bucket; parents contain children
-    L, B = inf.leaf_slots, inf.leaf_size
-    for j in range(0, inf.num_leaves, 211):
-        seg = P[j * B:(j + 1) * B]
-        assert np.array_equal(box[L - 1 + j, 0], seg.min(0))
-        assert np.array_equal(box[L - 1 + j, 1], seg.max(0))
-    inner = np.arange(L - 1)
-    for c in (1, 2):
-        ch = 2 * inner + c
-        ok = box[ch, 0] <= box[ch, 1]
-        assert np.all((box[inner, 0] <= box[ch, 0])[ok])
-        assert np.all((box[inner, 1] >= box[ch, 1])[ok])
+    check_tree_structure(P, box, nodes, inf.leaf_size)
+
+
+def check_tree_structure(P, box, nodes, leaf):
+    """Every node's points are its range; children partition the parent's
+    range in order; leaves hold <= leaf points and cover every point once;
+    boxes are the tight boxes of their points."""
+    a, b, child, parent = nodes.T
+    nn = len(nodes)
+    assert a[0] == 0 and b[0] == len(P) and parent[0] == -1
+    inner = np.nonzero(child >= 0)[0]
+    c0, c1 = child[inner], child[inner] + 1
+    assert np.all(c0 % 2 == 1) and np.all(c1 < nn)
+    assert np.all(parent[c0] == inner) and np.all(parent[c1] == inner)
+    assert np.array_equal(a[c0], a[inner]) and np.array_equal(b[c1], b[inner])
+    assert np.array_equal(b[c0], a[c1]) and np.all(a[c0] < b[c0]) and np.all(a[c1] < b[c1])
+    leaves = np.nonzero(child < 0)[0]
+    assert np.all(b[leaves] - a[leaves] <= leaf)
+    assert (b[leaves] - a[leaves]).sum() == len(P)
+    assert np.all(b[inner] - a[inner] > leaf)
+    # each node is reachable from the root exactly once
+    seen = np.zeros(nn, np.int32)
+    seen[0] += 1
+    seen[c0] += 1
+    seen[c1] += 1
+    assert np.all(seen == 1)
+    # tight leaf boxes, parent boxes are unions of child boxes
+    for i in leaves[:: max(1, len(leaves) // 500)]:
+        seg = P[a[i]:b[i]]
+        assert np.array_equal(box[i, 0], seg.min(0)) and np.array_equal(box[i, 1], seg.max(0))
+    assert np.array_equal(box[inner, 0], np.minimum(box[c0, 0], box[c1, 0]))
+    assert np.array_equal(box[inner, 1], np.maximum(box[c0, 1], box[c1, 1]))
+
+
+def test_tree_structure_duplicates(gpu):
+    """Many identical points: equal Morton codes split at the middle."""
+    from pptk_amd import capi
+    pts = np.concatenate([np.zeros((5000, 3)), np.ones((3000, 3)),
+                          np.random.default_rng(0).random((100, 3))]).astype(np.float32)
+    for leaf in (1, 7, 64):
+        t = capi.Tree.build(pts, leaf=leaf)
+        P, rev, fwd, box, nodes = t.export()
+        check_tree_structure(P, box, nodes, leaf)
 
 
 # ------------------------------------------------------------------- kNN ----

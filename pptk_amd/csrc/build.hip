@@ -371,6 +371,7 @@ template <typename D, int DIM>
 __global__ __launch_bounds__(kBlock) void node_box_kernel(const PointRec<D, DIM>* __restrict__ rec,
                                                           const NodeRec* __restrict__ nodes,
                                                           BoxRec<D, DIM>* __restrict__ box,
+                                                          int32_t* __restrict__ leafof,
                                                           int64_t s, int64_t e) {
   const int64_t i = s + (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= e) return;
@@ -383,6 +384,7 @@ __global__ __launch_bounds__(kBlock) void node_box_kernel(const PointRec<D, DIM>
   }
   if (nr.child < 0) {
     for (int32_t p = nr.a; p < nr.b; ++p) {
+      leafof[p] = (int32_t)i;
       const PointRec<D, DIM> r = rec[p];
 #pragma unroll
       for (int d = 0; d < DIM; ++d) {
@@ -496,10 +498,11 @@ void build_impl(pptk_tree* t, const T* dpts, int64_t n, hipStream_t s) {
   PK_HIP(hipMemcpyAsync(t->node.get(), nodes.get(), (size_t)nn * sizeof(NodeRec),
                         hipMemcpyDeviceToDevice, s));
   t->box.reset((size_t)nn * sizeof(BoxRec<D, DIM>));
+  t->leafof.reset((size_t)std::max<int64_t>(nv, 1) * 4);
   for (auto it = levels.rbegin(); it != levels.rend(); ++it) {
     node_box_kernel<D, DIM><<<grid_for(it->second - it->first, kBlock), kBlock, 0, s>>>(
         t->pts.as<PointRec<D, DIM>>(), t->node.as<NodeRec>(), t->box.as<BoxRec<D, DIM>>(),
-        it->first, it->second);
+        t->leafof.as<int32_t>(), it->first, it->second);
     PK_LAUNCH_CHECK();
   }
   PK_HIP(hipStreamSynchronize(s));  // temporaries return to the pool
@@ -613,7 +616,8 @@ extern "C" pptk_status pptk_tree_get_info(const pptk_tree* tree, pptk_tree_info*
     out->depth = tree->depth;
     for (int d = 0; d < 4; ++d) { out->bbox_lo[d] = tree->lo[d]; out->bbox_hi[d] = tree->hi[d]; }
     out->device_bytes = (int64_t)(tree->pts.bytes() + tree->orig.bytes() + tree->fwd.bytes() +
-                                  tree->box.bytes() + tree->node.bytes());
+                                  tree->box.bytes() + tree->node.bytes() +
+                                  tree->leafof.bytes());
   });
 }
 

@@ -238,6 +238,7 @@ struct TreeView {
   const int32_t* fwd;           // num_input, original index -> position or -1
   const BoxRec<D, DIM>* box;    // nn node boxes
   const NodeRec* node;          // nn nodes
+  const int32_t* leafof;        // num_valid, tree position -> leaf node
   int32_t n;                    // num_valid
   int32_t leaf;                 // maximum points per leaf
   int32_t nn;                   // number of nodes

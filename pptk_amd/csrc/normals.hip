@@ -195,11 +195,13 @@ __device__ __forceinline__ void count_fallbacks(bool fb, unsigned long long* nfb
 // explicit rows (t = output row, rows[t] = original index).
 template <typename D>
 __device__ __forceinline__ void resolve_row(const TreeView<D, 3>& tv, const int32_t* rows, int64_t t,
-                                            D* q, int64_t& row, bool& ok) {
+                                            D* q, int64_t& row, int32_t& pos, bool& ok) {
   ok = true;
+  pos = -1;
   if (!rows) {
     const PointRec<D, 3> r = tv.pts[t];
     row = r.orig;
+    pos = (int32_t)t;
 #pragma unroll
     for (int d = 0; d < 3; ++d) q[d] = r.c[d];
   } else {
@@ -209,6 +211,7 @@ __device__ __forceinline__ void resolve_row(const TreeView<D, 3>& tv, const int3
     if (p < 0) {
       ok = false;
     } else {
+      pos = p;
       const PointRec<D, 3> r = tv.pts[p];
 #pragma unroll
       for (int d = 0; d < 3; ++d) q[d] = r.c[d];
@@ -228,14 +231,15 @@ __global__ __launch_bounds__(kBlock) void normals_knn_kernel(TreeView<D, 3> tv,
   if (t >= m) return;
   D q[3];
   int64_t row;
+  int32_t pos;
   bool ok;
-  resolve_row<D>(tv, rows, t, q, row, ok);
+  resolve_row<D>(tv, rows, t, q, row, pos, ok);
   TopK<D, K, true> st;
   st.init(k);
-  if (ok) knn_search<D, 3>(tv, q, r2, -1, st);
+  if (ok) knn_search_from_leaf<D, 3>(tv, pos, q, r2, -1, st);
   int n = 0;
 #pragma unroll
-  for (int s = 0; s < K; ++s) n += (s >= K - k && st.d[s] < pos_inf<D>()) ? 1 : 0;
+  for (int s = 0; s < K; ++s) n += (s >= K - k && st.dist(s) < pos_inf<D>()) ? 1 : 0;
   bool fb = false;
   if (n == 0) {
     write_nan<D, NE>(row, evecs, evals, sizes);
@@ -243,7 +247,7 @@ __global__ __launch_bounds__(kBlock) void normals_knn_kernel(TreeView<D, 3> tv,
     auto each = [&](auto f) {
 #pragma unroll
       for (int s = 0; s < K; ++s)
-        if (s >= K - k && st.d[s] < pos_inf<D>()) f(st.p[s]);
+        if (s >= K - k && st.dist(s) < pos_inf<D>()) f(st.p[s]);
     };
     double cd[3][3];
     float cf[3][3];
@@ -268,12 +272,13 @@ __global__ __launch_bounds__(kBlock) void normals_heap_kernel(TreeView<D, 3> tv,
   if (t >= t1) return;
   D q[3];
   int64_t row;
+  int32_t pos;
   bool ok;
-  resolve_row<D>(tv, rows, t, q, row, ok);
+  resolve_row<D>(tv, rows, t, q, row, pos, ok);
   const int64_t w0 = (t - t0) * k;
   HeapK<D> st;
   st.init(ws_d + w0, ws_o + w0, ws_p + w0, k);
-  if (ok) knn_search<D, 3>(tv, q, r2, -1, st);
+  if (ok) knn_search_from_leaf<D, 3>(tv, pos, q, r2, -1, st);
   const int n = st.size;
   bool fb = false;
   if (n == 0) {
@@ -306,8 +311,9 @@ __global__ __launch_bounds__(kBlock) void normals_rnear_kernel(TreeView<D, 3> tv
   if (t >= m) return;
   D q[3];
   int64_t row;
+  int32_t pos;
   bool ok;
-  resolve_row<D>(tv, rows, t, q, row, ok);
+  resolve_row<D>(tv, rows, t, q, row, pos, ok);
   int64_t n = 0;
   double s1[3] = {0, 0, 0};
   double s2[6] = {0, 0, 0, 0, 0, 0};

@@ -38,15 +38,19 @@ __global__ __launch_bounds__(kBlock) void knn_kernel(TreeView<D, DIM> tv, int mo
   if (!resolve_query<D, DIM>(tv, mode, qlist, qpts, t, nrows, q, excl, row, ok)) return;
   TopK<D, K, false> st;
   st.init(k);
-  if (ok) knn_search<D, DIM>(tv, q, r2, excl, st);
+  if (ok) {
+    if (excl >= 0) knn_search_from_leaf<D, DIM>(tv, excl, q, r2, excl, st);
+    else knn_search<D, DIM>(tv, q, r2, excl, st);
+  }
   int32_t c = 0;
   int32_t* orow = out_idx + row * k;
 #pragma unroll
   for (int s = 0; s < K; ++s) {
     if (s >= K - k) {
-      const bool f = st.d[s] < pos_inf<D>();
-      orow[s - (K - k)] = f ? st.o[s] : -1;
-      if (out_dist) out_dist[row * k + (s - (K - k))] = f ? st.d[s] : pos_inf<D>();
+      const D ds = st.dist(s);
+      const bool f = ds < pos_inf<D>();
+      orow[s - (K - k)] = f ? st.idx(s) : -1;
+      if (out_dist) out_dist[row * k + (s - (K - k))] = f ? ds : pos_inf<D>();
       c += f ? 1 : 0;
     }
   }
@@ -73,7 +77,10 @@ __global__ __launch_bounds__(kBlock) void knn_large_kernel(TreeView<D, DIM> tv, 
   int32_t* ho = out_idx + row * k;
   HeapK<D> st;
   st.init(hd, ho, nullptr, k);
-  if (ok) knn_search<D, DIM>(tv, q, r2, excl, st);
+  if (ok) {
+    if (excl >= 0) knn_search_from_leaf<D, DIM>(tv, excl, q, r2, excl, st);
+    else knn_search<D, DIM>(tv, q, r2, excl, st);
+  }
   st.finish();
   for (int s = st.size; s < k; ++s) ho[s] = -1;
   out_cnt[row] = st.size;
@@ -81,6 +88,31 @@ __global__ __launch_bounds__(kBlock) void knn_large_kernel(TreeView<D, DIM> tv, 
     for (int s = 0; s < k; ++s) out_dist[row * k + s] = s < st.size ? hd[s] : pos_inf<D>();
 }
 
+__device__ __forceinline__ int32_t lane_read(int32_t x, int l) {
+  return __builtin_amdgcn_readlane(x, l);
+}
+__device__ __forceinline__ int64_t lane_read64(int64_t x, int l) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int32_t)(uint32_t)x, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int32_t)(uint32_t)((uint64_t)x >> 32), l);
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ float lane_read(float x, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+__device__ __forceinline__ double lane_read(double x, int l) {
+  return __longlong_as_double(lane_read64(__double_as_longlong(x), l));
+}
+
+enum : int { kItemNone = 0, kItemRange = 1, kItemLeaf = 2 };
+
+// r-near, count (FILL = false) or fill (FILL = true). Each lane walks its
+// query's tree in left-to-right order doing box tests only; whenever it
+// reaches a node wholly inside the ball (box upper bound < r*r,
+// RNearNeighborsHelper, kdtree-impl.h:527-529) or a leaf that straddles the
+// ball, it stops and posts that node as a work item. The wave then serves
+// every posted item with all 64 lanes: a range item is a coalesced copy of
+// original indices, a leaf item tests up to 64 points at once and compacts
+// the hits with a ballot. Output order within a query is tree order.
 template <typename D, int DIM, bool FILL>
 __global__ __launch_bounds__(kBlock) void rnear_kernel(TreeView<D, DIM> tv, int mode,
                                                        const int32_t* __restrict__ qlist,
@@ -90,18 +122,99 @@ __global__ __launch_bounds__(kBlock) void rnear_kernel(TreeView<D, DIM> tv, int 
                                                        const int64_t* __restrict__ offsets,
                                                        int32_t* __restrict__ out) {
   const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (t >= m) return;
+  const int lane = threadIdx.x & 63;
+  const uint64_t lt_mask = (1ull << lane) - 1ull;
   D q[DIM];
-  int32_t excl;
-  int64_t row;
-  bool ok;
-  if (!resolve_query<D, DIM>(tv, mode, qlist, qpts, t, nrows, q, excl, row, ok)) return;
-  if constexpr (FILL) {
-    if (!ok) return;
-    const int64_t o0 = offsets[row];
-    rnear_walk<D, DIM, true>(tv, q, r2, excl, out + o0, offsets[row + 1] - o0);
-  } else {
-    counts[row] = ok ? (int32_t)rnear_walk<D, DIM, false>(tv, q, r2, excl, nullptr, 0) : 0;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) q[d] = D(0);
+  int32_t excl = -1;
+  int64_t row = 0;
+  bool ok = false, has_row = false;
+  if (t < m) has_row = resolve_query<D, DIM>(tv, mode, qlist, qpts, t, nrows, q, excl, row, ok);
+  bool active = has_row && ok;
+  int64_t base = 0, cap = 0;
+  if (FILL && active) {
+    base = offsets[row];
+    cap = offsets[row + 1] - base;
+  }
+  int64_t cnt = 0;
+  int32_t node = 0;
+  // Every lane stays in the loop until the whole wave is done: the item
+  // service below is executed by all 64 lanes together.
+  while (__ballot(active)) {
+    int kind = kItemNone;
+    int32_t ia = 0, ib = 0;
+    if (active) {
+      for (;;) {
+        const BoxRec<D, DIM> b = tv.box[node];
+        if (box_min_dist2<D, DIM>(b, q) < r2) {
+          const NodeRec nr = tv.node[node];
+          if (box_max_dist2<D, DIM>(b, q) < r2) {
+            kind = kItemRange;
+            ia = nr.a;
+            ib = nr.b;
+          } else if (nr.child < 0) {
+            kind = kItemLeaf;
+            ia = nr.a;
+            ib = nr.b;
+          } else {
+            node = nr.child;
+            continue;
+          }
+        }
+        node = next_in_order<D, DIM>(tv, node);
+        if (node == 0) active = false;
+        if (kind != kItemNone || !active) break;
+      }
+    }
+    uint64_t pend = __ballot(kind != kItemNone);
+    while (pend) {
+      const int l = __builtin_ctzll(pend);
+      pend &= pend - 1ull;
+      const int32_t a = lane_read(ia, l);
+      const int32_t b = lane_read(ib, l);
+      const int32_t kd = lane_read((int32_t)kind, l);
+      const int32_t ex = lane_read(excl, l);
+      const int64_t done = lane_read64(cnt, l);
+      int64_t produced;
+      if (kd == kItemRange) {
+        const bool has_ex = ex >= a && ex < b;
+        produced = (int64_t)(b - a) - (has_ex ? 1 : 0);
+        if constexpr (FILL) {
+          const int64_t ob = lane_read64(base, l);
+          const int64_t lim = lane_read64(cap, l);
+          for (int32_t p = a + lane; p < b; p += 64) {
+            if (p == ex) continue;
+            const int64_t o = done + (p - a) - ((has_ex && p > ex) ? 1 : 0);
+            if (o < lim) out[ob + o] = tv.orig[p];
+          }
+        }
+      } else {
+        D ql[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) ql[d] = lane_read(q[d], l);
+        const int32_t p = a + lane;
+        bool pass = false;
+        int32_t og = 0;
+        if (p < b) {
+          const PointRec<D, DIM> r = tv.pts[p];
+          pass = dist2<D, DIM>(ql, r.c) < r2 && p != ex;
+          og = r.orig;
+        }
+        const uint64_t hits = __ballot(pass);
+        produced = __popcll(hits);
+        if constexpr (FILL) {
+          const int64_t ob = lane_read64(base, l);
+          const int64_t lim = lane_read64(cap, l);
+          const int64_t o = done + __popcll(hits & lt_mask);
+          if (pass && o < lim) out[ob + o] = og;
+        }
+      }
+      if (lane == l) cnt += produced;
+    }
+  }
+  if constexpr (!FILL) {
+    if (has_row) counts[row] = ok ? (int32_t)cnt : 0;
   }
 }
 

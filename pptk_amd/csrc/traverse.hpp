@@ -46,6 +46,8 @@ struct TopK {
     }
   }
   __device__ __forceinline__ D bound() const { return d[K - 1]; }
+  __device__ __forceinline__ D dist(int s) const { return d[s]; }
+  __device__ __forceinline__ int32_t idx(int s) const { return o[s]; }
   __device__ __forceinline__ bool beats(D dd, int32_t oo) const {
     return key_less(dd, oo, d[K - 1], o[K - 1]);
   }
@@ -62,6 +64,50 @@ struct TopK {
     const bool h0 = key_less(dd, oo, d[0], o[0]);
     d[0] = h0 ? dd : d[0];
     o[0] = h0 ? oo : o[0];
+    if constexpr (POS) p[0] = h0 ? pp : p[0];
+  }
+};
+
+// Float specialisation: (d, o) packed into one 64-bit key, d's IEEE bits
+// above the original index. Squared distances are +0 or positive, so the
+// unsigned key order equals the (d, o) order and each slot update is one
+// 64-bit compare. Sentinel slots hold key 0 (an inserted key 0 lands right
+// after them); empty slots hold (+inf, 0xffffffff).
+template <int K, bool POS>
+struct TopK<float, K, POS> {
+  static constexpr uint64_t kEmpty = (0x7f800000ull << 32) | 0xffffffffull;
+  uint64_t key[K];
+  int32_t p[POS ? K : 1];
+  __device__ __forceinline__ float dist(int s) const {
+    return __uint_as_float((uint32_t)(key[s] >> 32));
+  }
+  __device__ __forceinline__ int32_t idx(int s) const { return (int32_t)(uint32_t)key[s]; }
+
+  static __device__ __forceinline__ uint64_t make(float dd, int32_t oo) {
+    return ((uint64_t)__float_as_uint(dd) << 32) | (uint32_t)oo;
+  }
+  __device__ __forceinline__ void init(int k) {
+#pragma unroll
+    for (int s = 0; s < K; ++s) {
+      key[s] = s < K - k ? 0ull : kEmpty;
+      if constexpr (POS) p[s] = -1;
+    }
+  }
+  __device__ __forceinline__ float bound() const {
+    return __uint_as_float((uint32_t)(key[K - 1] >> 32));
+  }
+  __device__ __forceinline__ bool beats(float dd, int32_t oo) const { return make(dd, oo) < key[K - 1]; }
+  __device__ __forceinline__ void insert(float dd, int32_t oo, int32_t pp) {
+    const uint64_t kk = make(dd, oo);
+#pragma unroll
+    for (int s = K - 1; s > 0; --s) {
+      const bool sh = kk < key[s - 1];
+      const bool here = kk < key[s];
+      key[s] = sh ? key[s - 1] : (here ? kk : key[s]);
+      if constexpr (POS) p[s] = sh ? p[s - 1] : (here ? pp : p[s]);
+    }
+    const bool h0 = kk < key[0];
+    key[0] = h0 ? kk : key[0];
     if constexpr (POS) p[0] = h0 ? pp : p[0];
   }
 };
@@ -163,14 +209,14 @@ __device__ __forceinline__ bool get_flip(uint64_t lo, uint64_t hi, int depth) {
   return depth < 64 ? ((lo >> depth) & 1ull) : ((hi >> (depth - 64)) & 1ull);
 }
 
-// k-nearest search from the root, near child first.
+// k-nearest search inside the subtree rooted at `start`, near child first.
 template <typename D, int DIM, typename State>
-__device__ void knn_search(const TreeView<D, DIM>& tv, const D* q, D r2, int32_t excl, State& st) {
-  if (!(box_min_dist2<D, DIM>(tv.box[0], q) < r2)) return;
-  int32_t node = 0;
+__device__ void knn_subtree(const TreeView<D, DIM>& tv, int32_t start, const D* q, D r2,
+                            int32_t excl, State& st) {
+  int32_t node = start;
   int depth = 0;
   uint64_t flo = 0, fhi = 0;
-  NodeRec cur = tv.node[0];
+  NodeRec cur = tv.node[start];
   for (;;) {
     bool at_leaf = true;
     while (cur.child >= 0) {
@@ -207,56 +253,39 @@ __device__ void knn_search(const TreeView<D, DIM>& tv, const D* q, D r2, int32_t
   }
 }
 
+// k-nearest search from the root (queries given as points).
+template <typename D, int DIM, typename State>
+__device__ void knn_search(const TreeView<D, DIM>& tv, const D* q, D r2, int32_t excl, State& st) {
+  if (!(box_min_dist2<D, DIM>(tv.box[0], q) < r2)) return;
+  knn_subtree<D, DIM>(tv, 0, q, r2, excl, st);
+}
+
+// k-nearest search for a query that is the tree point at position `pos`:
+// scan its own leaf, then walk up, searching each ancestor's other child
+// while that child's box can still hold a better neighbour. Same result as
+// knn_search (every box that can hold a neighbour is still visited); the
+// nearest candidates are simply found first.
+template <typename D, int DIM, typename State>
+__device__ void knn_search_from_leaf(const TreeView<D, DIM>& tv, int32_t pos, const D* q, D r2,
+                                     int32_t excl, State& st) {
+  int32_t node = tv.leafof[pos];
+  NodeRec cur = tv.node[node];
+  knn_scan_leaf<D, DIM>(tv, cur.a, cur.b, q, r2, excl, st);
+  while (node != 0) {
+    const int32_t sib = (node & 1) ? node + 1 : node - 1;
+    const D ms = box_min_dist2<D, DIM>(tv.box[sib], q);
+    if (ms < r2 && ms <= st.bound()) knn_subtree<D, DIM>(tv, sib, q, r2, excl, st);
+    node = cur.parent;
+    cur = tv.node[node];
+  }
+}
+
 // Next node after finishing `node`'s subtree in left-to-right order, or 0
 // when the walk is complete.
 template <typename D, int DIM>
 __device__ __forceinline__ int32_t next_in_order(const TreeView<D, DIM>& tv, int32_t node) {
   while (node != 0 && (node & 1) == 0) node = tv.node[node].parent;
   return node == 0 ? 0 : node + 1;
-}
-
-// r-near walk in tree order (left subtree first). Counts (FILL = false) or
-// writes at most `cap` original indices to `out` (FILL = true). A subtree
-// whose box upper bound is < r*r is emitted whole, as
-// RNearNeighborsHelper does (kdtree-impl.h:527-529).
-template <typename D, int DIM, bool FILL>
-__device__ int64_t rnear_walk(const TreeView<D, DIM>& tv, const D* q, D r2, int32_t excl,
-                              int32_t* __restrict__ out, int64_t cap) {
-  int64_t cnt = 0;
-  int32_t node = 0;
-  for (;;) {
-    const BoxRec<D, DIM> b = tv.box[node];
-    if (box_min_dist2<D, DIM>(b, q) < r2) {
-      const NodeRec nr = tv.node[node];
-      if (box_max_dist2<D, DIM>(b, q) < r2) {
-        if constexpr (FILL) {
-          for (int32_t p = nr.a; p < nr.b; ++p) {
-            if (p == excl) continue;
-            if (cnt < cap) out[cnt] = tv.orig[p];
-            ++cnt;
-          }
-        } else {
-          cnt += (nr.b - nr.a) - ((excl >= nr.a && excl < nr.b) ? 1 : 0);
-        }
-      } else if (nr.child < 0) {
-        for (int32_t p = nr.a; p < nr.b; ++p) {
-          const PointRec<D, DIM> r = tv.pts[p];
-          if (dist2<D, DIM>(q, r.c) < r2 && p != excl) {
-            if constexpr (FILL) {
-              if (cnt < cap) out[cnt] = r.orig;
-            }
-            ++cnt;
-          }
-        }
-      } else {
-        node = nr.child;
-        continue;
-      }
-    }
-    node = next_in_order<D, DIM>(tv, node);
-    if (node == 0) break;
-  }
-  return cnt;
 }
 
 // Calls visit(record) for every point with d < r*r, in tree order (used by
@@ -296,7 +325,7 @@ __device__ __forceinline__ bool resolve_query(const TreeView<D, DIM>& tv, int mo
                                               int64_t nrows, D* q, int32_t& excl, int64_t& row,
                                               bool& ok) {
   ok = true;
-  excl = -1;
+  excl = -1;   // for self queries: the query's own tree position
   if (mode == kQueryAll) {
     const PointRec<D, DIM> r = tv.pts[t];
     row = r.orig;

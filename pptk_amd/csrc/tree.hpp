@@ -21,6 +21,7 @@ struct pptk_tree {
   pk::DevBuf fwd;   // int32[n_input]
   pk::DevBuf box;   // BoxRec<D,dim>[nn]
   pk::DevBuf node;  // NodeRec[nn]
+  pk::DevBuf leafof;  // int32[n_valid]: leaf node holding each tree position
 };
 
 namespace pk {
@@ -35,6 +36,7 @@ TreeView<D, DIM> tree_view(const pptk_tree* t) {
   v.fwd = t->fwd.as<const int32_t>();
   v.box = t->box.as<const BoxRec<D, DIM>>();
   v.node = t->node.as<const NodeRec>();
+  v.leafof = t->leafof.as<const int32_t>();
   v.n = (int32_t)t->n_valid;
   v.leaf = t->leaf;
   v.nn = (int32_t)t->nn;

@@ -113,7 +113,21 @@ def main():
     ap.add_argument("--leaf", type=int, default=0)
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB mebibytes (counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        nbytes = args.calibrate_copy << 20
+        a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda:0").fill_(1.0)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes,
+                          "copies": 3}), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU; --gpus %d is not supported (queries and trees "
               "shard by replication / by cloud, see DESIGN.md)" % args.gpus, file=sys.stderr)

@@ -27,6 +27,7 @@
 #include <set>
 #include <vector>
 
+#include "morton.hpp"
 #include "sort.hpp"
 #include "tree.hpp"
 
@@ -37,17 +38,6 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kPer = 16;                 // consecutive points per thread
 constexpr int kTile = kBlock * kPer;     // 4096
-
-template <int DIM> struct MortonBits { static constexpr int value = DIM == 2 ? 31 : DIM == 3 ? 21 : 15; };
-
-template <typename D>
-struct BuildInfo {
-  uint32_t n_valid;
-  uint32_t pad;
-  D lo[4];
-  D hi[4];
-  D scale[4];
-};
 
 template <typename T>
 __device__ __forceinline__ bool elem_finite(T v) {
@@ -216,51 +206,6 @@ __global__ __launch_bounds__(kBlock) void ingest_finalize_kernel(const uint32_t*
     }
     for (int d = 0; d < 4; ++d)
       info->scale[d] = (d < DIM && ext > D(0) && finite_val(ext)) ? maxq / ext : D(0);
-  }
-}
-
-__device__ __forceinline__ uint64_t spread3_21(uint32_t a) {
-  uint64_t x = a & 0x1fffffull;
-  x = (x | (x << 32)) & 0x1f00000000ffffull;
-  x = (x | (x << 16)) & 0x1f0000ff0000ffull;
-  x = (x | (x << 8)) & 0x100f00f00f00f00full;
-  x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
-  x = (x | (x << 2)) & 0x1249249249249249ull;
-  return x;
-}
-
-__device__ __forceinline__ uint64_t spread2_31(uint32_t a) {
-  uint64_t x = a & 0x7fffffffull;
-  x = (x | (x << 16)) & 0x0000ffff0000ffffull;
-  x = (x | (x << 8)) & 0x00ff00ff00ff00ffull;
-  x = (x | (x << 4)) & 0x0f0f0f0f0f0f0f0full;
-  x = (x | (x << 2)) & 0x3333333333333333ull;
-  x = (x | (x << 1)) & 0x5555555555555555ull;
-  return x;
-}
-
-template <typename D, int DIM>
-__device__ __forceinline__ uint64_t morton_code(const D* c, const BuildInfo<D>& in) {
-  constexpr int bits = MortonBits<DIM>::value;
-  const D maxq = (D)((1u << bits) - 1u);
-  uint32_t u[DIM];
-#pragma unroll
-  for (int d = 0; d < DIM; ++d) {
-    D x = (c[d] - in.lo[d]) * in.scale[d];
-    x = x > D(0) ? x : D(0);
-    x = x < maxq ? x : maxq;
-    u[d] = (uint32_t)x;
-  }
-  if constexpr (DIM == 3) {
-    return spread3_21(u[0]) | (spread3_21(u[1]) << 1) | (spread3_21(u[2]) << 2);
-  } else if constexpr (DIM == 2) {
-    return spread2_31(u[0]) | (spread2_31(u[1]) << 1);
-  } else {
-    uint64_t code = 0;
-    for (int b = 0; b < bits; ++b)
-#pragma unroll
-      for (int d = 0; d < DIM; ++d) code |= (uint64_t)((u[d] >> b) & 1u) << (b * DIM + d);
-    return code;
   }
 }
 
@@ -436,6 +381,8 @@ void build_impl(pptk_tree* t, const T* dpts, int64_t n, hipStream_t s) {
   for (int d = 0; d < 4; ++d) {
     t->lo[d] = d < DIM ? (double)info.lo[d] : 0.0;
     t->hi[d] = d < DIM ? (double)info.hi[d] : 0.0;
+    t->mlo[d] = (double)info.lo[d];
+    t->mscale[d] = (double)info.scale[d];
   }
 
   const int64_t B = t->leaf;

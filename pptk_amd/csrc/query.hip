@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <vector>
 
+#include "morton.hpp"
 #include "query.hpp"
 #include "sort.hpp"
 #include "traverse.hpp"
@@ -24,7 +25,8 @@ constexpr int kBlock = 256;
 template <typename D, int DIM, int K>
 __global__ __launch_bounds__(kBlock) void knn_kernel(TreeView<D, DIM> tv, int mode,
                                                      const int32_t* __restrict__ qlist,
-                                                     const D* __restrict__ qpts, int64_t m,
+                                                     const D* __restrict__ qpts,
+                                                     const int32_t* __restrict__ order, int64_t m,
                                                      int64_t nrows, int k, D r2,
                                                      int32_t* __restrict__ out_idx,
                                                      int32_t* __restrict__ out_cnt,
@@ -35,7 +37,7 @@ __global__ __launch_bounds__(kBlock) void knn_kernel(TreeView<D, DIM> tv, int mo
   int32_t excl;
   int64_t row;
   bool ok;
-  if (!resolve_query<D, DIM>(tv, mode, qlist, qpts, t, nrows, q, excl, row, ok)) return;
+  if (!resolve_query<D, DIM>(tv, mode, qlist, qpts, order, t, nrows, q, excl, row, ok)) return;
   TopK<D, K, false> st;
   st.init(k);
   if (ok) {
@@ -60,7 +62,9 @@ __global__ __launch_bounds__(kBlock) void knn_kernel(TreeView<D, DIM> tv, int mo
 template <typename D, int DIM>
 __global__ __launch_bounds__(kBlock) void knn_large_kernel(TreeView<D, DIM> tv, int mode,
                                                            const int32_t* __restrict__ qlist,
-                                                           const D* __restrict__ qpts, int64_t t0,
+                                                           const D* __restrict__ qpts,
+                                                           const int32_t* __restrict__ order,
+                                                           int64_t t0,
                                                            int64_t t1, int64_t nrows, int k, D r2,
                                                            int32_t* __restrict__ out_idx,
                                                            int32_t* __restrict__ out_cnt,
@@ -72,7 +76,7 @@ __global__ __launch_bounds__(kBlock) void knn_large_kernel(TreeView<D, DIM> tv, 
   int32_t excl;
   int64_t row;
   bool ok;
-  if (!resolve_query<D, DIM>(tv, mode, qlist, qpts, t, nrows, q, excl, row, ok)) return;
+  if (!resolve_query<D, DIM>(tv, mode, qlist, qpts, order, t, nrows, q, excl, row, ok)) return;
   D* hd = ws_d + (t - t0) * k;
   int32_t* ho = out_idx + row * k;
   HeapK<D> st;
@@ -116,8 +120,9 @@ enum : int { kItemNone = 0, kItemRange = 1, kItemLeaf = 2 };
 template <typename D, int DIM, bool FILL>
 __global__ __launch_bounds__(kBlock) void rnear_kernel(TreeView<D, DIM> tv, int mode,
                                                        const int32_t* __restrict__ qlist,
-                                                       const D* __restrict__ qpts, int64_t m,
-                                                       int64_t nrows, D r2,
+                                                       const D* __restrict__ qpts,
+                                                       const int32_t* __restrict__ order,
+                                                       int64_t m, int64_t nrows, D r2,
                                                        int32_t* __restrict__ counts,
                                                        const int64_t* __restrict__ offsets,
                                                        int32_t* __restrict__ out) {
@@ -130,7 +135,7 @@ __global__ __launch_bounds__(kBlock) void rnear_kernel(TreeView<D, DIM> tv, int 
   int32_t excl = -1;
   int64_t row = 0;
   bool ok = false, has_row = false;
-  if (t < m) has_row = resolve_query<D, DIM>(tv, mode, qlist, qpts, t, nrows, q, excl, row, ok);
+  if (t < m) has_row = resolve_query<D, DIM>(tv, mode, qlist, qpts, order, t, nrows, q, excl, row, ok);
   bool active = has_row && ok;
   int64_t base = 0, cap = 0;
   if (FILL && active) {
@@ -238,6 +243,77 @@ template void fill_async<int32_t>(int32_t*, int64_t, int32_t, hipStream_t);
 template void fill_async<float>(float*, int64_t, float, hipStream_t);
 template void fill_async<double>(double*, int64_t, double, hipStream_t);
 
+namespace {
+
+template <typename D, int DIM>
+__global__ __launch_bounds__(kBlock) void query_morton_kernel(const D* __restrict__ qpts, int64_t m,
+                                                              BuildInfo<D> frame,
+                                                              uint64_t* __restrict__ keys,
+                                                              uint32_t* __restrict__ vals) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= m) return;
+  D c[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) c[d] = qpts[t * DIM + d];
+  keys[t] = morton_code<D, DIM>(c, frame);
+  vals[t] = (uint32_t)t;
+}
+
+__global__ __launch_bounds__(kBlock) void query_position_kernel(const int32_t* __restrict__ list,
+                                                                const int32_t* __restrict__ fwd,
+                                                                int64_t m, int64_t n_input,
+                                                                uint64_t* __restrict__ keys,
+                                                                uint32_t* __restrict__ vals) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= m) return;
+  const int32_t o = list[t];
+  const int32_t p = (o >= 0 && (int64_t)o < n_input) ? fwd[o] : -1;
+  keys[t] = p < 0 ? 0xffffffffull : (uint64_t)(uint32_t)p;
+  vals[t] = (uint32_t)t;
+}
+
+// Orders point queries along the tree's Morton curve and index queries by
+// tree position. Below this many queries the sort costs more than it saves.
+constexpr int64_t kOrderMin = 4096;
+
+template <typename D>
+void order_queries(const pptk_tree* t, PreparedQueries& pq, hipStream_t s) {
+  const int64_t m = pq.m;
+  if (m < kOrderMin || pq.mode == kQueryAll) return;
+  DevBuf keys((size_t)m * 8);
+  pq.order_buf.reset((size_t)m * 4);
+  int bits = 32;
+  if (pq.mode == kQueryList) {
+    query_position_kernel<<<grid_of(m), kBlock, 0, s>>>(
+        pq.dlist, t->fwd.as<const int32_t>(), m, t->n_input, keys.as<uint64_t>(),
+        pq.order_buf.as<uint32_t>());
+    PK_LAUNCH_CHECK();
+  } else {
+    BuildInfo<D> frame;
+    frame.n_valid = 0;
+    frame.pad = 0;
+    for (int d = 0; d < 4; ++d) {
+      frame.lo[d] = (D)t->mlo[d];
+      frame.hi[d] = (D)t->hi[d];
+      frame.scale[d] = (D)t->mscale[d];
+    }
+    auto launch = [&](auto dimc) {
+      constexpr int DIM = decltype(dimc)::value;
+      query_morton_kernel<D, DIM><<<grid_of(m), kBlock, 0, s>>>(
+          (const D*)pq.dpts, m, frame, keys.as<uint64_t>(), pq.order_buf.as<uint32_t>());
+      PK_LAUNCH_CHECK();
+      bits = DIM * MortonBits<DIM>::value;
+    };
+    if (t->dim == 2) launch(std::integral_constant<int, 2>());
+    else if (t->dim == 3) launch(std::integral_constant<int, 3>());
+    else launch(std::integral_constant<int, 4>());
+  }
+  radix_sort_pairs(keys.as<uint64_t>(), pq.order_buf.as<uint32_t>(), m, bits, s);
+  pq.order = pq.order_buf.as<const int32_t>();
+}
+
+}  // namespace
+
 template <typename D>
 void prepare_queries(const pptk_tree* t, const pptk_queries* q, hipStream_t s, PreparedQueries& pq) {
   if (!q) fail(PPTK_E_VALUE, "null query set");
@@ -291,6 +367,7 @@ void prepare_queries(const pptk_tree* t, const pptk_queries* q, hipStream_t s, P
     default:
       fail(PPTK_E_VALUE, "unknown query kind %d", (int)q->kind);
   }
+  order_queries<D>(t, pq, s);
 }
 template void prepare_queries<float>(const pptk_tree*, const pptk_queries*, hipStream_t, PreparedQueries&);
 template void prepare_queries<double>(const pptk_tree*, const pptk_queries*, hipStream_t, PreparedQueries&);
@@ -324,7 +401,7 @@ void knn_run(const pptk_tree* t, const pptk_queries* qs, int k, double dmax, int
       auto launch = [&](auto kc) {
         constexpr int K = decltype(kc)::value;
         knn_kernel<D, DIM, K><<<grid_of(m), kBlock, 0, s>>>(
-            tv, pq.mode, pq.dlist, qp, m, nrows, k, r2, (int32_t*)oi.dptr, (int32_t*)oc.dptr,
+            tv, pq.mode, pq.dlist, qp, pq.order, m, nrows, k, r2, (int32_t*)oi.dptr, (int32_t*)oc.dptr,
             (D*)od.dptr);
         PK_LAUNCH_CHECK();
       };
@@ -339,7 +416,7 @@ void knn_run(const pptk_tree* t, const pptk_queries* qs, int k, double dmax, int
       for (int64_t t0 = 0; t0 < m; t0 += chunk) {
         const int64_t t1 = std::min(m, t0 + chunk);
         knn_large_kernel<D, DIM><<<grid_of(t1 - t0), kBlock, 0, s>>>(
-            tv, pq.mode, pq.dlist, qp, t0, t1, nrows, k, r2, (int32_t*)oi.dptr,
+            tv, pq.mode, pq.dlist, qp, pq.order, t0, t1, nrows, k, r2, (int32_t*)oi.dptr,
             (int32_t*)oc.dptr, ws.as<D>(), (D*)od.dptr);
         PK_LAUNCH_CHECK();
       }
@@ -367,7 +444,8 @@ void rnear_count_run(const pptk_tree* t, const pptk_queries* qs, double dmax, in
   const TreeView<D, DIM> tv = tree_view<D, DIM>(t);
   if (pq.m > 0) {
     rnear_kernel<D, DIM, false><<<grid_of(pq.m), kBlock, 0, s>>>(
-        tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.m, nrows, r2, counts.as<int32_t>(),
+        tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.order, pq.m, nrows, r2,
+        counts.as<int32_t>(),
         nullptr, nullptr);
     PK_LAUNCH_CHECK();
   }
@@ -407,7 +485,7 @@ void rnear_fill_run(const pptk_tree* t, const pptk_queries* qs, double dmax, con
   const TreeView<D, DIM> tv = tree_view<D, DIM>(t);
   if (pq.m > 0 && total > 0) {
     rnear_kernel<D, DIM, true><<<grid_of(pq.m), kBlock, 0, s>>>(
-        tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.m, nrows, r2, nullptr,
+        tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.order, pq.m, nrows, r2, nullptr,
         (const int64_t*)doff.ptr, (int32_t*)oi.dptr);
     PK_LAUNCH_CHECK();
   }

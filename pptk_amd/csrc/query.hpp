@@ -12,6 +12,10 @@ struct PreparedQueries {
   const int32_t* dlist = nullptr;
   DevBuf conv;
   const void* dpts = nullptr;
+  // Processing order (thread t serves query order[t]) so that neighbouring
+  // lanes get nearby queries; null = input order.
+  DevBuf order_buf;
+  const int32_t* order = nullptr;
 };
 
 template <typename D>

@@ -321,11 +321,13 @@ __device__ void rnear_visit(const TreeView<D, DIM>& tv, const D* q, D r2, V& vis
 template <typename D, int DIM>
 __device__ __forceinline__ bool resolve_query(const TreeView<D, DIM>& tv, int mode,
                                               const int32_t* __restrict__ qlist,
-                                              const D* __restrict__ qpts, int64_t t,
+                                              const D* __restrict__ qpts,
+                                              const int32_t* __restrict__ order, int64_t t,
                                               int64_t nrows, D* q, int32_t& excl, int64_t& row,
                                               bool& ok) {
   ok = true;
   excl = -1;   // for self queries: the query's own tree position
+  if (order && mode != kQueryAll) t = order[t];
   if (mode == kQueryAll) {
     const PointRec<D, DIM> r = tv.pts[t];
     row = r.orig;

@@ -16,6 +16,8 @@ struct pptk_tree {
   int64_t nn = 1;         // number of nodes
   double lo[4] = {0, 0, 0, 0};
   double hi[4] = {0, 0, 0, 0};
+  double mlo[4] = {0, 0, 0, 0};     // Morton frame (exact copies of the
+  double mscale[4] = {0, 0, 0, 0};  // distance-type values used in the build)
   pk::DevBuf pts;   // PointRec<D,dim>[n_valid]
   pk::DevBuf orig;  // int32[n_valid]
   pk::DevBuf fwd;   // int32[n_input]

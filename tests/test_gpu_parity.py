@@ -184,7 +184,7 @@ def test_knn_index_queries(gpu):
     kd = _kd()
     tree = kd._build(pts)
     ot = O.OracleTree(pts)
-    idx = rng.integers(-n, n, 3000)
+    idx = rng.integers(-n, n, 6000)
     got, _ = _rows(kd._query(tree, list(idx), k=7))
     want, _ = ot.knn(np.where(idx < 0, idx + n, idx), 7)
     assert np.array_equal(got, want)
@@ -293,7 +293,7 @@ def test_rnear_self_and_points(gpu, dtype):
         assert len(res) == len(off) - 1
         for i in range(0, len(res), 7):
             assert np.array_equal(np.sort(res[i]), idx[off[i]:off[i + 1]]), i
-        q = rng.random((3000, 3))
+        q = rng.random((6000, 3))
         res = kd._query(tree, q, k=None, dmax=dmax)
         off, idx = ot.rnear(q.astype(ot.dtype), r)
         for i in range(len(res)):

@@ -42,15 +42,38 @@ def log(*a):
 def algorithmic_bytes(n, k, m_rnear, rnear_total):
     s = 4  # float32 coordinates
     return {
-        # read points, write reordered records' coordinates + permutation
+        # read points, write reordered coordinates + permutation
         "build": n * (3 * s + 3 * s + 4),
         # read query coordinates, write k int32 indices + 1 int32 count
         "knn": n * (3 * s + 4 * k + 4),
-        # read query, write one int64 offset and the int32 neighbour ids
-        "rnear": m_rnear * (3 * s + 8) + 4 * rnear_total,
+        # read query, write its int32 count and int64 offset
+        "rnear_count": m_rnear * (3 * s + 4 + 8),
+        # read query and its offsets, write the int32 neighbour ids
+        "rnear_fill": m_rnear * (3 * s + 16) + 4 * rnear_total,
         # read point, write the unit normal
         "normals": n * (3 * s + 3 * s),
     }
+
+
+# The kernel that dominates each phase (names as rocprofv3 reports them).
+PHASE_KERNEL = {
+    "knn": "knn_kernel<float, 3, 16>",
+    "rnear_count": "rnear_kernel<float, 3, false>",
+    "rnear_fill": "rnear_kernel<float, 3, true>",
+    "normals": "normals_knn_kernel<float, 16, 1, 1>",
+}
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from profiles/traffic.json (rocprofv3
+    FETCH_SIZE and WRITE_SIZE runs scaled by a plain-copy calibration)."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)["kernels"][kernel]
+        return t["fetch_bytes"] + t["write_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def cpu_baseline(pts, qpts, k, dmax):
@@ -162,7 +185,7 @@ def main():
     qpt, _ = capi.make_queries(dev_ptr=q_d.data_ptr(), m=m, kind=capi.Q_POINTS,
                                dtype=np.float32)
     prec = capi.PREC_MIXED if args.precision == "mixed" else capi.PREC_FP64
-    phases = ["build", "knn", "rnear", "normals"]
+    phases = ["build", "knn", "rnear_count", "rnear_fill", "normals"]
     state = {"rtotal": 0, "nfb": 0}
 
     def step(record):
@@ -181,6 +204,7 @@ def main():
             capi.check(L.pptk_rnear_count(ctypes.c_void_p(tree.handle), ctypes.byref(qpt), dmax,
                                           ctypes.c_void_p(off_d.data_ptr()), ctypes.byref(total),
                                           capi.MEM_DEVICE, sh))
+            ev[3].record(stream)
             if ridx_d is None or ridx_d.numel() < total.value:
                 ridx_d = torch.empty(max(1, int(total.value * 1.1)), dtype=torch.int32,
                                      device=dev)
@@ -188,12 +212,12 @@ def main():
                                          ctypes.c_void_p(off_d.data_ptr()),
                                          ctypes.c_void_p(ridx_d.data_ptr()), capi.MEM_DEVICE,
                                          sh))
-            ev[3].record(stream)
+            ev[4].record(stream)
             nfb = ctypes.c_int64()
             capi.check(L.pptk_normals(ctypes.c_void_p(tree.handle), k, float("inf"), None, 0, 1,
                                       prec, ctypes.c_void_p(nrm_d.data_ptr()), None, None,
                                       ctypes.byref(nfb), capi.MEM_DEVICE, sh))
-            ev[4].record(stream)
+            ev[5].record(stream)
         tree.free()
         state["rtotal"] = int(total.value)
         state["nfb"] = int(nfb.value)
@@ -228,8 +252,9 @@ def main():
     log("bench: kNN spot check on 2000 rows vs oracle: %s" % ("exact" if exact else "MISMATCH"))
 
     ab = algorithmic_bytes(n, k, m, state["rtotal"])
-    dom = max(["knn", "normals"], key=lambda p: ph[p])
+    dom = max(PHASE_KERNEL, key=lambda p: ph[p])
     achieved = ab[dom] / (ph[dom] * 1e-3) / 1e9
+    traffic = measured_traffic(PHASE_KERNEL[dom])
     out = {
         "metric": METRIC,
         "value": 2 * n / (ms_step * 1e-3),
@@ -254,19 +279,19 @@ def main():
         "phases_ms": ph,
         "knn_qps": n / (ph["knn"] * 1e-3),
         "normals_ps": n / (ph["normals"] * 1e-3),
-        "rnear_qps": m / (ph["rnear"] * 1e-3),
+        "rnear_qps": m / ((ph["rnear_count"] + ph["rnear_fill"]) * 1e-3),
         "build_pts_per_s": n / (ph["build"] * 1e-3),
         "rnear_neighbours": state["rtotal"],
         "normals_fp64_fallback_frac": state["nfb"] / n,
         "knn_spot_check_exact": exact,
         "roofline": {
             "bound": "hbm",
-            "kernel": dom,
+            "kernel": PHASE_KERNEL[dom],
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
             "algorithmic_bytes_per_launch": ab[dom],
         },
     }

@@ -92,31 +92,45 @@ __global__ __launch_bounds__(kBlock) void knn_large_kernel(TreeView<D, DIM> tv, 
     for (int s = 0; s < k; ++s) out_dist[row * k + s] = s < st.size ? hd[s] : pos_inf<D>();
 }
 
-__device__ __forceinline__ int32_t lane_read(int32_t x, int l) {
-  return __builtin_amdgcn_readlane(x, l);
-}
-__device__ __forceinline__ int64_t lane_read64(int64_t x, int l) {
-  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int32_t)(uint32_t)x, l);
-  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int32_t)(uint32_t)((uint64_t)x >> 32), l);
-  return (int64_t)(((uint64_t)hi << 32) | lo);
-}
-__device__ __forceinline__ float lane_read(float x, int l) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
-}
-__device__ __forceinline__ double lane_read(double x, int l) {
-  return __longlong_as_double(lane_read64(__double_as_longlong(x), l));
-}
-
 enum : int { kItemNone = 0, kItemRange = 1, kItemLeaf = 2 };
 
-// r-near, count (FILL = false) or fill (FILL = true). Each lane walks its
-// query's tree in left-to-right order doing box tests only; whenever it
-// reaches a node wholly inside the ball (box upper bound < r*r,
-// RNearNeighborsHelper, kdtree-impl.h:527-529) or a leaf that straddles the
-// ball, it stops and posts that node as a work item. The wave then serves
-// every posted item with all 64 lanes: a range item is a coalesced copy of
-// original indices, a leaf item tests up to 64 points at once and compacts
-// the hits with a ballot. Output order within a query is tree order.
+__device__ __forceinline__ int32_t wave_incl_scan_i32(int32_t x) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int32_t y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  return x;
+}
+
+// Lane whose segment [incl - size, incl) of a wave-wide inclusive prefix
+// `incl` contains slot s (0 <= s < total): the smallest lane with incl > s.
+__device__ __forceinline__ int owner_of(int32_t incl, int32_t s) {
+  int lo = 0, hi = 63;
+#pragma unroll
+  for (int it = 0; it < 6; ++it) {
+    const int mid = (lo + hi) >> 1;
+    const int32_t v = __shfl(incl, mid);
+    if (v > s) hi = mid;
+    else lo = mid + 1;
+  }
+  return lo;
+}
+
+// r-near, count (FILL = false) or fill (FILL = true).
+//
+// Each lane walks its query's tree in left-to-right order doing box tests
+// only, and stops at the next work item: a node wholly inside the ball (box
+// upper bound < r*r, RNearNeighborsHelper, kdtree-impl.h:527-529) or a leaf
+// that straddles the ball. Every round, each lane posts one item and the
+// wave serves all posted items together: a wave-wide prefix sum over the
+// item sizes spreads their elements over the 64 lanes, each lane finds its
+// element's owner by binary search over the prefix, and
+//   * range elements are copied (fill) or just counted (count: sizes only);
+//   * leaf elements are distance-tested against the owner's query; hits are
+//     ranked within the owner's segment with a ballot.
+// Output order within a query is tree order, identical in both passes.
 template <typename D, int DIM, bool FILL>
 __global__ __launch_bounds__(kBlock) void rnear_kernel(TreeView<D, DIM> tv, int mode,
                                                        const int32_t* __restrict__ qlist,
@@ -135,21 +149,31 @@ __global__ __launch_bounds__(kBlock) void rnear_kernel(TreeView<D, DIM> tv, int 
   int32_t excl = -1;
   int64_t row = 0;
   bool ok = false, has_row = false;
-  if (t < m) has_row = resolve_query<D, DIM>(tv, mode, qlist, qpts, order, t, nrows, q, excl, row, ok);
+  if (t < m)
+    has_row = resolve_query<D, DIM>(tv, mode, qlist, qpts, order, t, nrows, q, excl, row, ok);
   bool active = has_row && ok;
   int64_t base = 0, cap = 0;
   if (FILL && active) {
     base = offsets[row];
     cap = offsets[row + 1] - base;
   }
-  int64_t cnt = 0;
+  int32_t cnt = 0;  // outputs so far for this lane's query
   int32_t node = 0;
+  // A range item is posted at most kMaxRange elements at a time (the rest is
+  // posted in the next rounds), which keeps the wave-wide prefix in int32.
+  constexpr int32_t kMaxRange = 1 << 24;
+  int32_t rest_a = 0, rest_b = 0;
   // Every lane stays in the loop until the whole wave is done: the item
   // service below is executed by all 64 lanes together.
-  while (__ballot(active)) {
+  while (__ballot(active || rest_b > rest_a)) {
     int kind = kItemNone;
     int32_t ia = 0, ib = 0;
-    if (active) {
+    if (rest_b > rest_a) {
+      kind = kItemRange;
+      ia = rest_a;
+      ib = rest_b - rest_a > kMaxRange ? rest_a + kMaxRange : rest_b;
+      rest_a = ib;
+    } else if (active) {
       for (;;) {
         const BoxRec<D, DIM> b = tv.box[node];
         if (box_min_dist2<D, DIM>(b, q) < r2) {
@@ -171,55 +195,94 @@ __global__ __launch_bounds__(kBlock) void rnear_kernel(TreeView<D, DIM> tv, int 
         if (node == 0) active = false;
         if (kind != kItemNone || !active) break;
       }
+      if (kind == kItemRange && ib - ia > kMaxRange) {
+        rest_a = ia + kMaxRange;
+        rest_b = ib;
+        ib = rest_a;
+      }
     }
-    uint64_t pend = __ballot(kind != kItemNone);
-    while (pend) {
-      const int l = __builtin_ctzll(pend);
-      pend &= pend - 1ull;
-      const int32_t a = lane_read(ia, l);
-      const int32_t b = lane_read(ib, l);
-      const int32_t kd = lane_read((int32_t)kind, l);
-      const int32_t ex = lane_read(excl, l);
-      const int64_t done = lane_read64(cnt, l);
-      int64_t produced;
-      if (kd == kItemRange) {
-        const bool has_ex = ex >= a && ex < b;
-        produced = (int64_t)(b - a) - (has_ex ? 1 : 0);
-        if constexpr (FILL) {
-          const int64_t ob = lane_read64(base, l);
-          const int64_t lim = lane_read64(cap, l);
-          for (int32_t p = a + lane; p < b; p += 64) {
-            if (p == ex) continue;
-            const int64_t o = done + (p - a) - ((has_ex && p > ex) ? 1 : 0);
-            if (o < lim) out[ob + o] = tv.orig[p];
+    const bool has_ex = excl >= ia && excl < ib;
+    // ---- ranges: every element except the query itself is an output ----
+    if (kind == kItemRange) cnt += (ib - ia) - (has_ex ? 1 : 0);
+    if constexpr (FILL) {
+      const int32_t esz = kind == kItemRange ? ib - ia : 0;
+      const int32_t incl = wave_incl_scan_i32(esz);
+      const int32_t tot = __shfl(incl, 63);
+      // this lane's outputs before the current range item
+      const int32_t start = kind == kItemRange ? cnt - ((ib - ia) - (has_ex ? 1 : 0)) : cnt;
+      for (int32_t s0 = 0; s0 < tot; s0 += 64) {
+        const int32_t sl = s0 + lane;
+        const int src = owner_of(incl, sl < tot ? sl : tot - 1);
+        const int32_t o_incl = __shfl(incl, src);
+        const int32_t o_a = __shfl(ia, src);
+        const int32_t o_b = __shfl(ib, src);
+        const int32_t o_ex = __shfl(excl, src);
+        const int32_t o_start = __shfl(start, src);
+        const int64_t o_base = __shfl(base, src);
+        const int64_t o_cap = __shfl(cap, src);
+        if (sl < tot) {
+          const int32_t p = o_a + (sl - (o_incl - (o_b - o_a)));
+          if (p != o_ex) {
+            const bool ex_in = o_ex >= o_a && o_ex < o_b;
+            const int64_t o = (int64_t)o_start + (p - o_a) - ((ex_in && p > o_ex) ? 1 : 0);
+            if (o < o_cap) out[o_base + o] = tv.orig[p];
           }
         }
-      } else {
-        D ql[DIM];
+      }
+    }
+    // ---- leaves: test every point against its owner's query ----
+    {
+      const int32_t lsz = kind == kItemLeaf ? ib - ia : 0;
+      const int32_t incl = wave_incl_scan_i32(lsz);
+      const int32_t tot = __shfl(incl, 63);
+      const int32_t seg0 = incl - lsz;  // this lane's first slot
+      int32_t hits_mine = 0;
+      for (int32_t s0 = 0; s0 < tot; s0 += 64) {
+        const int32_t sl = s0 + lane;
+        const int src = owner_of(incl, sl < tot ? sl : tot - 1);
+        const int32_t o_seg0 = __shfl(seg0, src);
+        const int32_t o_a = __shfl(ia, src);
+        const int32_t o_ex = __shfl(excl, src);
+        D oq[DIM];
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) ql[d] = lane_read(q[d], l);
-        const int32_t p = a + lane;
+        for (int d = 0; d < DIM; ++d) oq[d] = __shfl(q[d], src);
         bool pass = false;
         int32_t og = 0;
-        if (p < b) {
+        const int32_t p = o_a + (sl - o_seg0);
+        if (sl < tot) {
           const PointRec<D, DIM> r = tv.pts[p];
-          pass = dist2<D, DIM>(ql, r.c) < r2 && p != ex;
+          pass = dist2<D, DIM>(oq, r.c) < r2 && p != o_ex;
           og = r.orig;
         }
         const uint64_t hits = __ballot(pass);
-        produced = __popcll(hits);
         if constexpr (FILL) {
-          const int64_t ob = lane_read64(base, l);
-          const int64_t lim = lane_read64(cap, l);
-          const int64_t o = done + __popcll(hits & lt_mask);
-          if (pass && o < lim) out[ob + o] = og;
+          // rank among the owner's hits: earlier chunks + earlier lanes of
+          // the owner's segment in this chunk
+          const int32_t lo_lane = o_seg0 > s0 ? o_seg0 - s0 : 0;
+          const uint64_t seg_lt = lt_mask & ~((1ull << lo_lane) - 1ull);
+          const int32_t o_done = __shfl(hits_mine, src);
+          const int32_t o_cnt = __shfl(cnt, src);
+          const int64_t o_base = __shfl(base, src);
+          const int64_t o_cap = __shfl(cap, src);
+          const int64_t o = (int64_t)o_cnt + o_done + __popcll(hits & seg_lt);
+          if (pass && o < o_cap) out[o_base + o] = og;
+        }
+        // this lane's hits in this chunk (its segment [seg0, incl))
+        if (lsz > 0) {
+          const int32_t a0 = seg0 > s0 ? seg0 - s0 : 0;
+          const int32_t a1 = incl < s0 + 64 ? incl - s0 : 64;
+          if (a1 > a0) {
+            const uint64_t hi_mask = a1 >= 64 ? ~0ull : ((1ull << a1) - 1ull);
+            const uint64_t mine = hi_mask & ~((1ull << a0) - 1ull);
+            hits_mine += __popcll(hits & mine);
+          }
         }
       }
-      if (lane == l) cnt += produced;
+      cnt += hits_mine;
     }
   }
   if constexpr (!FILL) {
-    if (has_row) counts[row] = ok ? (int32_t)cnt : 0;
+    if (has_row) counts[row] = ok ? cnt : 0;
   }
 }
 

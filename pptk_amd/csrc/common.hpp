@@ -243,6 +243,8 @@ struct TreeView {
   int32_t leaf;                 // maximum points per leaf
   int32_t nn;                   // number of nodes
   int64_t n_input;
+  const int32_t* n_dev;         // if set, num_valid is read from device memory
+  int32_t nroots;               // roots are nodes [0, nroots) (one per cloud)
 };
 
 }  // namespace pk

@@ -199,6 +199,11 @@ __device__ __forceinline__ void resolve_row(const TreeView<D, 3>& tv, const int3
   ok = true;
   pos = -1;
   if (!rows) {
+    if (tv.n_dev && t >= *tv.n_dev) {
+      ok = false;
+      row = -1;
+      return;
+    }
     const PointRec<D, 3> r = tv.pts[t];
     row = r.orig;
     pos = (int32_t)t;
@@ -234,6 +239,7 @@ __global__ __launch_bounds__(kBlock) void normals_knn_kernel(TreeView<D, 3> tv,
   int32_t pos;
   bool ok;
   resolve_row<D>(tv, rows, t, q, row, pos, ok);
+  if (row < 0) return;
   TopK<D, K, true> st;
   st.init(k);
   if (ok) knn_search_from_leaf<D, 3>(tv, pos, q, r2, -1, st);
@@ -275,6 +281,7 @@ __global__ __launch_bounds__(kBlock) void normals_heap_kernel(TreeView<D, 3> tv,
   int32_t pos;
   bool ok;
   resolve_row<D>(tv, rows, t, q, row, pos, ok);
+  if (row < 0) return;
   const int64_t w0 = (t - t0) * k;
   HeapK<D> st;
   st.init(ws_d + w0, ws_o + w0, ws_p + w0, k);
@@ -314,6 +321,7 @@ __global__ __launch_bounds__(kBlock) void normals_rnear_kernel(TreeView<D, 3> tv
   int32_t pos;
   bool ok;
   resolve_row<D>(tv, rows, t, q, row, pos, ok);
+  if (row < 0) return;
   int64_t n = 0;
   double s1[3] = {0, 0, 0};
   double s2[6] = {0, 0, 0, 0, 0, 0};
@@ -339,7 +347,7 @@ __global__ __launch_bounds__(kBlock) void normals_rnear_kernel(TreeView<D, 3> tv
         s2[3] = s2[3] + e[1] * e[1]; s2[4] = s2[4] + e[1] * e[2]; s2[5] = s2[5] + e[2] * e[2];
       }
     };
-    rnear_visit<D, 3>(tv, q, r2, visit);
+    rnear_visit<D, 3>(tv, root_of<D, 3>(tv, pos), q, r2, visit);
   }
   bool fb = false;
   if (n == 0) {

@@ -159,6 +159,7 @@ __global__ __launch_bounds__(kBlock) void rnear_kernel(TreeView<D, DIM> tv, int 
   }
   int32_t cnt = 0;  // outputs so far for this lane's query
   int32_t node = 0;
+  if (active && tv.nroots > 1) node = root_of<D, DIM>(tv, excl);
   // A range item is posted at most kMaxRange elements at a time (the rest is
   // posted in the next rounds), which keeps the wave-wide prefix in int32.
   constexpr int32_t kMaxRange = 1 << 24;
@@ -192,7 +193,7 @@ __global__ __launch_bounds__(kBlock) void rnear_kernel(TreeView<D, DIM> tv, int 
           }
         }
         node = next_in_order<D, DIM>(tv, node);
-        if (node == 0) active = false;
+        if (node < 0) active = false;
         if (kind != kItemNone || !active) break;
       }
       if (kind == kItemRange && ib - ia > kMaxRange) {
@@ -402,6 +403,9 @@ void prepare_queries(const pptk_tree* t, const pptk_queries* q, hipStream_t s, P
       break;
     }
     case PPTK_Q_POINTS: {
+      if (t->nroots > 1)
+        fail(PPTK_E_VALUE, "point queries need a single-cloud tree (this tree has %lld clouds)",
+             (long long)t->nroots);
       pq.mode = kQueryPoints;
       if (q->m < 0) fail(PPTK_E_VALUE, "negative query count");
       pq.m = pq.nrows = q->m;

@@ -270,39 +270,48 @@ void exclusive_scan(const Tin* in, Tout* out, int64_t n, hipStream_t s) {
 template void exclusive_scan<int32_t, int64_t>(const int32_t*, int64_t*, int64_t, hipStream_t);
 template void exclusive_scan<uint32_t, uint32_t>(const uint32_t*, uint32_t*, int64_t, hipStream_t);
 
-void radix_sort_pairs(uint64_t* keys, uint32_t* vals, int64_t n, int end_bit, hipStream_t s) {
+void SortWorkspace::reserve(int64_t n) {
+  if (n <= n_cap) return;
+  const int64_t tiles = ceil_div(std::max<int64_t>(n, 1), kTile);
+  const int64_t hlen = (int64_t)kRadix * tiles;
+  const int64_t scan_tiles = ceil_div(hlen, kScanTile);
+  kalt.reset((size_t)n * sizeof(uint64_t));
+  valt.reset((size_t)n * sizeof(uint32_t));
+  hist.reset((size_t)hlen * sizeof(uint32_t));
+  offs.reset((size_t)(hlen + 1) * sizeof(uint32_t));
+  ssum.reset((size_t)(scan_tiles + 1) * sizeof(uint32_t));
+  n_cap = n;
+}
+
+void radix_sort_pairs_async(uint64_t* keys, uint32_t* vals, int64_t n, int end_bit,
+                            SortWorkspace& ws, hipStream_t s) {
   if (n <= 1 || end_bit <= 0) return;
   if (n > (int64_t)0x7fffffff) fail(PPTK_E_VALUE, "radix sort: n=%lld too large", (long long)n);
+  if (n > ws.n_cap) fail(PPTK_E_RUNTIME, "radix sort workspace too small");
   const int64_t tiles = ceil_div(n, kTile);
-  DevBuf kalt((size_t)n * sizeof(uint64_t));
-  DevBuf valt((size_t)n * sizeof(uint32_t));
   const int64_t hlen = (int64_t)kRadix * tiles;
-  DevBuf hist((size_t)hlen * sizeof(uint32_t));
-  DevBuf offs((size_t)(hlen + 1) * sizeof(uint32_t));
   const int64_t scan_tiles = ceil_div(hlen, kScanTile);
-  DevBuf ssum((size_t)(scan_tiles + 1) * sizeof(uint32_t));
-
   uint64_t* k0 = keys;
   uint32_t* v0 = vals;
-  uint64_t* k1 = kalt.as<uint64_t>();
-  uint32_t* v1 = valt.as<uint32_t>();
+  uint64_t* k1 = ws.kalt.as<uint64_t>();
+  uint32_t* v1 = ws.valt.as<uint32_t>();
+  uint32_t* hist = ws.hist.as<uint32_t>();
+  uint32_t* offs = ws.offs.as<uint32_t>();
+  uint32_t* ts = ws.ssum.as<uint32_t>();
   int passes = 0;
   for (int shift = 0; shift < end_bit; shift += 8, ++passes) {
-    radix_hist_kernel<<<dim3((unsigned)tiles), dim3(kBlock), 0, s>>>(k0, n, shift,
-                                                                     hist.as<uint32_t>(),
-                                                                     (int)tiles);
+    radix_hist_kernel<<<dim3((unsigned)tiles), dim3(kBlock), 0, s>>>(k0, n, shift, hist, (int)tiles);
     PK_LAUNCH_CHECK();
-    uint32_t* ts = ssum.as<uint32_t>();
     scan_reduce_kernel<uint32_t, uint32_t>
-        <<<dim3((unsigned)scan_tiles), dim3(kScanBlock), 0, s>>>(hist.as<uint32_t>(), hlen, ts);
+        <<<dim3((unsigned)scan_tiles), dim3(kScanBlock), 0, s>>>(hist, hlen, ts);
     PK_LAUNCH_CHECK();
     scan_single_kernel<uint32_t><<<1, kScanBlock, 0, s>>>(ts, scan_tiles);
     PK_LAUNCH_CHECK();
     scan_down_kernel<uint32_t, uint32_t><<<dim3((unsigned)scan_tiles), dim3(kScanBlock), 0, s>>>(
-        hist.as<uint32_t>(), offs.as<uint32_t>(), hlen, ts);
+        hist, offs, hlen, ts);
     PK_LAUNCH_CHECK();
-    radix_scatter_kernel<<<dim3((unsigned)tiles), dim3(kBlock), 0, s>>>(
-        k0, v0, k1, v1, n, shift, offs.as<uint32_t>(), (int)tiles);
+    radix_scatter_kernel<<<dim3((unsigned)tiles), dim3(kBlock), 0, s>>>(k0, v0, k1, v1, n, shift,
+                                                                        offs, (int)tiles);
     PK_LAUNCH_CHECK();
     uint64_t* tk = k0; k0 = k1; k1 = tk;
     uint32_t* tv = v0; v0 = v1; v1 = tv;
@@ -311,7 +320,19 @@ void radix_sort_pairs(uint64_t* keys, uint32_t* vals, int64_t n, int end_bit, hi
     PK_HIP(hipMemcpyAsync(keys, k0, (size_t)n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
     PK_HIP(hipMemcpyAsync(vals, v0, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
   }
-  PK_HIP(hipStreamSynchronize(s));
+}
+
+void radix_sort_pairs(uint64_t* keys, uint32_t* vals, int64_t n, int end_bit, hipStream_t s) {
+  if (n <= 1 || end_bit <= 0) return;
+  SortWorkspace ws;
+  ws.reserve(n);
+  radix_sort_pairs_async(keys, vals, n, end_bit, ws, s);
+  PK_HIP(hipStreamSynchronize(s));  // the workspace returns to the pool
+}
+
+void scan_u32_inplace_async(uint32_t* data, int64_t n, hipStream_t s) {
+  scan_single_kernel<uint32_t><<<1, kScanBlock, 0, s>>>(data, n);
+  PK_LAUNCH_CHECK();
 }
 
 }  // namespace pk

@@ -179,20 +179,25 @@ struct HeapK {
 template <typename D, int DIM, typename State>
 __device__ __forceinline__ void knn_scan_leaf(const TreeView<D, DIM>& tv, int32_t a, int32_t b,
                                               const D* q, D r2, int32_t excl, State& st) {
-  const D kb = st.bound();
-  uint64_t mask = 0;
-  for (int32_t p = a; p < b; ++p) {
-    const PointRec<D, DIM> r = tv.pts[p];
-    const D dd = dist2<D, DIM>(q, r.c);
-    const bool pass = (dd < r2) && (dd <= kb) && (p != excl);
-    mask |= (uint64_t)pass << (p - a);
-  }
-  while (mask) {
-    const int s = __builtin_ctzll(mask);
-    mask &= mask - 1ull;
-    const PointRec<D, DIM> r = tv.pts[a + s];
-    const D dd = dist2<D, DIM>(q, r.c);
-    if (st.beats(dd, r.orig)) st.insert(dd, r.orig, a + s);
+  // Leaves hold at most 64 points unless a capped build stopped splitting;
+  // scan in chunks of 64 so the candidate mask always fits.
+  for (int32_t c0 = a; c0 < b; c0 += 64) {
+    const int32_t c1 = b - c0 > 64 ? c0 + 64 : b;
+    const D kb = st.bound();
+    uint64_t mask = 0;
+    for (int32_t p = c0; p < c1; ++p) {
+      const PointRec<D, DIM> r = tv.pts[p];
+      const D dd = dist2<D, DIM>(q, r.c);
+      const bool pass = (dd < r2) && (dd <= kb) && (p != excl);
+      mask |= (uint64_t)pass << (p - c0);
+    }
+    while (mask) {
+      const int s = __builtin_ctzll(mask);
+      mask &= mask - 1ull;
+      const PointRec<D, DIM> r = tv.pts[c0 + s];
+      const D dd = dist2<D, DIM>(q, r.c);
+      if (st.beats(dd, r.orig)) st.insert(dd, r.orig, c0 + s);
+    }
   }
 }
 
@@ -271,7 +276,7 @@ __device__ void knn_search_from_leaf(const TreeView<D, DIM>& tv, int32_t pos, co
   int32_t node = tv.leafof[pos];
   NodeRec cur = tv.node[node];
   knn_scan_leaf<D, DIM>(tv, cur.a, cur.b, q, r2, excl, st);
-  while (node != 0) {
+  while (cur.parent >= 0) {
     const int32_t sib = (node & 1) ? node + 1 : node - 1;
     const D ms = box_min_dist2<D, DIM>(tv.box[sib], q);
     if (ms < r2 && ms <= st.bound()) knn_subtree<D, DIM>(tv, sib, q, r2, excl, st);
@@ -280,19 +285,36 @@ __device__ void knn_search_from_leaf(const TreeView<D, DIM>& tv, int32_t pos, co
   }
 }
 
-// Next node after finishing `node`'s subtree in left-to-right order, or 0
-// when the walk is complete.
+// Next node after finishing `node`'s subtree in left-to-right order, or -1
+// when the walk has left the tree (a root has parent -1).
 template <typename D, int DIM>
 __device__ __forceinline__ int32_t next_in_order(const TreeView<D, DIM>& tv, int32_t node) {
-  while (node != 0 && (node & 1) == 0) node = tv.node[node].parent;
-  return node == 0 ? 0 : node + 1;
+  for (;;) {
+    const int32_t par = tv.node[node].parent;
+    if (par < 0) return -1;
+    if (node & 1) return node + 1;  // left child: its sibling is next
+    node = par;
+  }
+}
+
+// Root of the tree (cloud) holding tree position `pos`.
+template <typename D, int DIM>
+__device__ __forceinline__ int32_t root_of(const TreeView<D, DIM>& tv, int32_t pos) {
+  if (tv.nroots <= 1) return 0;
+  int32_t node = tv.leafof[pos];
+  for (;;) {
+    const int32_t par = tv.node[node].parent;
+    if (par < 0) return node;
+    node = par;
+  }
 }
 
 // Calls visit(record) for every point with d < r*r, in tree order (used by
 // the r-near normals; the query point itself is included).
 template <typename D, int DIM, typename V>
-__device__ void rnear_visit(const TreeView<D, DIM>& tv, const D* q, D r2, V& visit) {
-  int32_t node = 0;
+__device__ void rnear_visit(const TreeView<D, DIM>& tv, int32_t root, const D* q, D r2,
+                            V& visit) {
+  int32_t node = root;
   for (;;) {
     const BoxRec<D, DIM> b = tv.box[node];
     if (box_min_dist2<D, DIM>(b, q) < r2) {
@@ -310,7 +332,7 @@ __device__ void rnear_visit(const TreeView<D, DIM>& tv, const D* q, D r2, V& vis
       }
     }
     node = next_in_order<D, DIM>(tv, node);
-    if (node == 0) break;
+    if (node < 0) break;
   }
 }
 
@@ -329,6 +351,7 @@ __device__ __forceinline__ bool resolve_query(const TreeView<D, DIM>& tv, int mo
   excl = -1;   // for self queries: the query's own tree position
   if (order && mode != kQueryAll) t = order[t];
   if (mode == kQueryAll) {
+    if (tv.n_dev && t >= *tv.n_dev) return false;
     const PointRec<D, DIM> r = tv.pts[t];
     row = r.orig;
     if (row >= nrows) return false;

@@ -14,6 +14,7 @@ struct pptk_tree {
   int depth = 0;          // number of node levels
   int64_t nleaves = 0;
   int64_t nn = 1;         // number of nodes
+  int64_t nroots = 1;     // one root per cloud (forest builds)
   double lo[4] = {0, 0, 0, 0};
   double hi[4] = {0, 0, 0, 0};
   double mlo[4] = {0, 0, 0, 0};     // Morton frame (exact copies of the
@@ -43,6 +44,8 @@ TreeView<D, DIM> tree_view(const pptk_tree* t) {
   v.leaf = t->leaf;
   v.nn = (int32_t)t->nn;
   v.n_input = t->n_input;
+  v.n_dev = nullptr;
+  v.nroots = (int32_t)t->nroots;
   return v;
 }
 

@@ -207,6 +207,69 @@ pptk_status pptk_normals(const pptk_tree* tree, int k, double r, const int32_t* 
                          void* out_evecs, void* out_evals, int32_t* out_sizes,
                          int64_t* out_num_fp64, pptk_mem mem, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Batched clouds. No counterpart in pptk, where each cloud is one
+ * kdtree._build plus kdtree._query(None, k) plus estimate_normals call
+ * (kdtree_wrapper.cpp:281-304, kdtree-impl.h:957-976,
+ * estimate_normals.cpp:64-111). A batch plan fixes the cloud sizes and
+ * allocates every device buffer once; each run builds one tree per cloud
+ * (a forest over the concatenated points), runs the self-query kNN of every
+ * point within its cloud and the PCA normals of every point, on one stream,
+ * with no host synchronisation inside. With use_graph the whole run is
+ * captured once into a hip graph and replayed by later runs with the same
+ * buffers. Results per cloud are those of the single-cloud calls on that
+ * cloud alone: kNN rows identical, normals from the same neighbourhood.
+ * ------------------------------------------------------------------------ */
+typedef struct pptk_batch pptk_batch;
+
+typedef struct {
+  int k;               /* self-query kNN k, 0..32 (0 = no kNN output) */
+  int normals_k;       /* normals neighbourhood size incl. the point, 0..33
+                          (0 = no normals); dim must be 3 */
+  double dmax;         /* distance bound of both (INFINITY = none) */
+  int max_leaf_size;   /* bucket size 1..64, 0 = tuned default */
+  int precision;       /* pptk_precision for the normals */
+  int use_graph;       /* 1 = capture the run into a hip graph and replay it */
+  int query_block;     /* threads per block of the query kernel: 64, 128 or
+                          256 (0 = 256) */
+} pptk_batch_params;
+
+typedef struct {
+  int64_t num_clouds;
+  int64_t num_points;    /* total input rows */
+  int64_t num_valid;     /* rows without NaN/inf (after a run) */
+  int64_t num_nodes;     /* forest nodes (after a run) */
+  int depth;             /* deepest node level + 1 (after a run) */
+  int leaf_size;
+  int morton_bits;       /* quantisation bits per axis inside a cloud */
+  int graph_nodes;       /* kernel launches per run */
+  int64_t device_bytes;  /* device memory held by the plan */
+  int64_t num_fp64;      /* rows that took the FP64 eigensolve (last run) */
+} pptk_batch_info;
+
+/* Plan for clouds of the given sizes (host array of num_clouds counts,
+ * total at most 2^30). dim in {2,3,4}; dtype PPTK_F32 or PPTK_F64. */
+pptk_status pptk_batch_create(const int64_t* cloud_sizes, int64_t num_clouds, int dim,
+                              pptk_dtype dtype, const pptk_batch_params* params,
+                              pptk_batch** out);
+
+/* One run. All pointers are device pointers:
+ *   pts      : num_points x dim, clouds concatenated in order
+ *   out_idx  : num_points x k int32, neighbour indices LOCAL to the cloud,
+ *              ascending by (d, index), -1 padded; NULL if k == 0
+ *   out_cnt  : NULL or num_points int32 neighbour counts
+ *   out_nrm  : num_points x 3, eigenvector of the smallest eigenvalue (dtype
+ *              of the points), NaN for invalid rows; NULL if normals_k == 0
+ *   out_eval : NULL or num_points smallest eigenvalues
+ * Returns after the run completes on `stream` (NULL = library stream). */
+pptk_status pptk_batch_run(pptk_batch* b, const void* pts, int32_t* out_idx, int32_t* out_cnt,
+                           void* out_nrm, void* out_eval, void* stream);
+
+pptk_status pptk_batch_get_info(const pptk_batch* b, pptk_batch_info* out);
+
+/* Free a plan (and its graph). NULL is a no-op. */
+void pptk_batch_free(pptk_batch* b);
+
 /* Building blocks exported for testing and for callers that need them. */
 
 /* Stable ascending sort of (key, value) pairs by the key bits

@@ -36,6 +36,7 @@ EXPORTED = [
     "pptk_last_error", "pptk_device_count", "pptk_trim", "pptk_tree_build", "pptk_tree_free",
     "pptk_tree_check", "pptk_tree_get_info", "pptk_tree_export", "pptk_knn", "pptk_rnear_count",
     "pptk_rnear_fill", "pptk_normals", "pptk_sort_pairs_u64", "pptk_exclusive_scan_i32",
+    "pptk_batch_create", "pptk_batch_run", "pptk_batch_get_info", "pptk_batch_free",
 ]
 
 
@@ -56,6 +57,20 @@ class TreeInfo(ctypes.Structure):
                 ("num_nodes", ctypes.c_int64), ("depth", ctypes.c_int),
                 ("bbox_lo", ctypes.c_double * 4), ("bbox_hi", ctypes.c_double * 4),
                 ("device_bytes", ctypes.c_int64)]
+
+
+class BatchParams(ctypes.Structure):
+    _fields_ = [("k", ctypes.c_int), ("normals_k", ctypes.c_int), ("dmax", ctypes.c_double),
+                ("max_leaf_size", ctypes.c_int), ("precision", ctypes.c_int),
+                ("use_graph", ctypes.c_int), ("query_block", ctypes.c_int)]
+
+
+class BatchInfo(ctypes.Structure):
+    _fields_ = [("num_clouds", ctypes.c_int64), ("num_points", ctypes.c_int64),
+                ("num_valid", ctypes.c_int64), ("num_nodes", ctypes.c_int64),
+                ("depth", ctypes.c_int), ("leaf_size", ctypes.c_int),
+                ("morton_bits", ctypes.c_int), ("graph_nodes", ctypes.c_int),
+                ("device_bytes", ctypes.c_int64), ("num_fp64", ctypes.c_int64)]
 
 
 _lib = None
@@ -95,6 +110,14 @@ def lib():
     L.pptk_sort_pairs_u64.argtypes = [vp, vp, i64, ci, ci]
     L.pptk_exclusive_scan_i32.restype = ci
     L.pptk_exclusive_scan_i32.argtypes = [vp, vp, i64, ci]
+    L.pptk_batch_create.restype = ci
+    L.pptk_batch_create.argtypes = [vp, i64, ci, ci, ctypes.POINTER(BatchParams),
+                                    ctypes.POINTER(vp)]
+    L.pptk_batch_run.restype = ci
+    L.pptk_batch_run.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+    L.pptk_batch_get_info.restype = ci
+    L.pptk_batch_get_info.argtypes = [vp, ctypes.POINTER(BatchInfo)]
+    L.pptk_batch_free.argtypes = [vp]
     _lib = L
     return L
 
@@ -273,3 +296,51 @@ def normals(tree, k, r, rows=None, num_eigen=1, precision=PREC_AUTO, want_evals=
                              0 if rr is None else rr.shape[0], int(num_eigen), int(precision),
                              _p(ev), _p(ew), _p(sz), ctypes.byref(nfb), MEM_HOST, None))
     return ev, ew, sz, int(nfb.value)
+
+
+class Batch:
+    """Owning handle of a batch plan (pptk_batch_*): fixed cloud sizes, every
+    device buffer allocated up front, runs on device pointers."""
+
+    def __init__(self, sizes, *, dim=3, dtype=np.float32, k=16, normals_k=16, dmax=np.inf,
+                 leaf=0, precision=PREC_AUTO, use_graph=True, query_block=0):
+        sz = np.ascontiguousarray(np.asarray(sizes, np.int64))
+        self.sizes = sz
+        self.offsets = np.concatenate([[0], np.cumsum(sz)]).astype(np.int64)
+        self.dim = int(dim)
+        self.dtype = np.dtype(dtype)
+        self.k = int(k)
+        self.normals_k = int(normals_k)
+        prm = BatchParams(self.k, self.normals_k, float(dmax), int(leaf), int(precision),
+                          1 if use_graph else 0, int(query_block))
+        h = ctypes.c_void_p()
+        self.handle = None
+        check(lib().pptk_batch_create(_p(sz), sz.shape[0], self.dim, dtype_code(self.dtype),
+                                      ctypes.byref(prm), ctypes.byref(h)))
+        self.handle = h.value
+
+    @property
+    def num_points(self):
+        return int(self.offsets[-1])
+
+    def run(self, pts_ptr, idx_ptr=None, cnt_ptr=None, nrm_ptr=None, eval_ptr=None,
+            stream=None):
+        """One run on device pointers (ints); see pptk_batch_run."""
+        check(lib().pptk_batch_run(ctypes.c_void_p(self.handle), _p(pts_ptr), _p(idx_ptr),
+                                   _p(cnt_ptr), _p(nrm_ptr), _p(eval_ptr), stream))
+
+    def info(self):
+        inf = BatchInfo()
+        check(lib().pptk_batch_get_info(ctypes.c_void_p(self.handle), ctypes.byref(inf)))
+        return inf
+
+    def free(self):
+        if self.handle:
+            lib().pptk_batch_free(ctypes.c_void_p(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

@@ -42,13 +42,16 @@ template <typename D, int DIM>
 __device__ __forceinline__ uint64_t morton_code(const D* c, const BuildInfo<D>& in) {
   constexpr int bits = MortonBits<DIM>::value;
   const D maxq = (D)((1u << bits) - 1u);
+  // maxq rounds up in float for 31 bits; clamp again as an integer.
+  constexpr uint32_t umax = (1u << bits) - 1u;
   uint32_t u[DIM];
 #pragma unroll
   for (int d = 0; d < DIM; ++d) {
     D x = (c[d] - in.lo[d]) * in.scale[d];
     x = x > D(0) ? x : D(0);
     x = x < maxq ? x : maxq;
-    u[d] = (uint32_t)x;
+    const uint32_t ux = (uint32_t)x;
+    u[d] = ux < umax ? ux : umax;
   }
   if constexpr (DIM == 3) {
     return spread3_21(u[0]) | (spread3_21(u[1]) << 1) | (spread3_21(u[2]) << 2);

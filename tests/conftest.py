@@ -20,8 +20,20 @@ def gpu_count():
         return 0
 
 
+def _init_torch_first():
+    """Tests that hand torch device buffers to the library need torch's HIP
+    runtime initialised before the library's own: torch cannot start its
+    runtime once the library's has claimed the device in this process."""
+    try:
+        import torch
+    except Exception:
+        return
+    torch.cuda.is_available()
+
+
 @pytest.fixture(scope="session")
 def gpu():
+    _init_torch_first()
     if gpu_count() < 1:
         pytest.fail("no GPU visible to libpptk_amd")
     return True

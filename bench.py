@@ -16,7 +16,14 @@ launches on. The CPU baseline runs pptk's own k-d tree (compiled from the
 reference sources, oracle/_ref) or, when that is absent, the oracle's
 restatement, on a bounded sample, on the host cores.
 
+With --workload batched: 4096 float32 clouds (sizes log-uniform in
+[1e3, 2e5], seed 3; pptk_amd.scenes.batched_clouds); one step = one replay
+of the batch plan's hip graph (forest build + fused kNN k=16 + normals k=16
+for every point); the plain single-cloud path runs over the same clouds for
+comparison; bucket size and query block width come from pptk_amd.tune.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--no-cpu]
+                       [--workload lidar|batched]
 """
 import argparse
 import ctypes
@@ -130,8 +137,205 @@ def cpu_baseline(pts, qpts, k, dmax, gpu_knn_rows=None):
     }
 
 
+BATCH_KERNEL = "batch_query_kernel<float, 3, 16, 1, true>"
+
+
+def cpu_baseline_batched(clouds, k, cores):
+    """pptk's CPU path per cloud (kdtree._build + self kNN + normals) on every
+    64th cloud: pptk KdTree from the reference sources when built, the oracle
+    port for the normals."""
+    from oracle import oracle as O
+    kind = "reference" if O.ref_available() else "port"
+    sample = clouds[::64]
+    npts = sum(len(c) for c in sample)
+    t0 = time.perf_counter()
+    for c in sample:
+        tree = O.RefTree(c) if kind == "reference" else O.OracleTree(c)
+        if kind == "reference":
+            tree.knn(None, k, nthreads=cores, m_self=len(c))
+        else:
+            tree.knn(np.arange(len(c)), k, nthreads=cores)
+        del tree
+        O.OracleTree(c).normals(k, np.inf, nthreads=cores)
+    t = time.perf_counter() - t0
+    return {
+        "value": npts / t,
+        "unit": "points/s (kNN k=%d self query + normals k=%d per point)" % (k, k),
+        "cores": cores,
+        "kind": kind,
+        "sample": ("every 64th cloud (%d clouds, %d points): %s build + kNN, oracle-port "
+                   "normals (FP64 Jacobi)" % (len(sample), npts,
+                                              "pptk KdTree" if kind == "reference"
+                                              else "oracle port")),
+    }
+
+
+def bench_batched(args):
+    """Batched clouds: one plan over all clouds (forest build + fused kNN and
+    normals, one hip-graph replay per step) against the plain single-cloud
+    path (pptk_tree_build + pptk_knn + pptk_normals per cloud) on the same
+    clouds."""
+    import torch
+
+    from pptk_amd import capi, scenes, tune
+
+    k = args.k
+    log("bench: generating %d batched clouds (seed 3)" % args.clouds)
+    clouds = scenes.batched_clouds(args.clouds, seed=3)
+    sizes = np.array([len(c) for c in clouds], np.int64)
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    N = int(off[-1])
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.Stream(device=dev)
+    sh = stream.cuda_stream
+    pts_d = torch.from_numpy(np.concatenate(clouds)).to(dev)
+    idx_d = torch.empty((N, k), dtype=torch.int32, device=dev)
+    nrm_d = torch.empty((N, 3), dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+
+    # autotune on every 8th cloud
+    sel = np.arange(0, len(sizes), 8)
+    sample_d = torch.cat([pts_d[off[c]:off[c + 1]] for c in sel])
+    torch.cuda.synchronize()
+    log("bench: autotuning on %d clouds (%d points)" % (len(sel), sample_d.shape[0]))
+    if args.leaf:
+        best, table = {"leaf": args.leaf, "block": 256, "ms": None}, []
+    else:
+        best, table = tune.tune_batch(sample_d, sizes[sel], k=k, normals_k=k, stream=stream,
+                                      log=log, use_cache=False)
+    del sample_d
+    log("bench: tuned leaf %d block %d" % (best["leaf"], best["block"]))
+
+    plan = capi.Batch(sizes, k=k, normals_k=k, leaf=best["leaf"], query_block=best["block"],
+                      use_graph=True)
+    run_args = (pts_d.data_ptr(), idx_d.data_ptr(), None, nrm_d.data_ptr(), None)
+    for _ in range(args.warmup):
+        plan.run(*run_args, stream=sh)
+    torch.cuda.synchronize()
+    times = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        plan.run(*run_args, stream=sh)
+        e1.record(stream)
+        stream.synchronize()
+        times.append(e0.elapsed_time(e1))
+        log("bench: batched step %d %.2f ms" % (i, times[-1]))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_step = wall * 1e3 / args.steps
+    binfo = plan.info()
+
+    # phase split from one eager timed run (same kernels, no graph)
+    tplan = capi.Batch(sizes, k=k, normals_k=k, leaf=best["leaf"], query_block=best["block"],
+                       use_graph=False, timing=True)
+    tplan.run(*run_args, stream=sh)
+    tinfo = tplan.info()
+    tplan.free()
+
+    if args.no_plain:
+        plan.free()
+        print(json.dumps({"workload": "batched", "ms_per_step": ms_step,
+                          "phases_ms_eager": {"build": tinfo.build_ms,
+                                              "query": tinfo.query_ms}}), flush=True)
+        return
+    # plain single-cloud path on the same clouds
+    log("bench: plain single-cloud path over %d clouds" % len(sizes))
+    L = capi.lib()
+    pidx_d = torch.empty((N, k), dtype=torch.int32, device=dev)
+    pcnt_d = torch.empty(N, dtype=torch.int32, device=dev)
+    pnrm_d = torch.empty((N, 3), dtype=torch.float32, device=dev)
+    qall, _ = capi.make_queries(None, mem=capi.MEM_DEVICE)
+    qall.kind = capi.Q_ALL
+    shp = ctypes.c_void_p(sh)
+
+    def plain(c):
+        a, n = int(off[c]), int(sizes[c])
+        tree = capi.Tree.build(dev_ptr=pts_d[a:].data_ptr(), n=n, dim=3, dtype=np.float32,
+                               leaf=0, stream=shp)
+        capi.check(L.pptk_knn(ctypes.c_void_p(tree.handle), ctypes.byref(qall), k,
+                              float("inf"), ctypes.c_void_p(pidx_d[a:].data_ptr()),
+                              ctypes.c_void_p(pcnt_d[a:].data_ptr()), None, capi.MEM_DEVICE,
+                              shp))
+        capi.check(L.pptk_normals(ctypes.c_void_p(tree.handle), k, float("inf"), None, 0, 1,
+                                  capi.PREC_MIXED, ctypes.c_void_p(pnrm_d[a:].data_ptr()), None,
+                                  None, None, capi.MEM_DEVICE, shp))
+        tree.free()
+
+    for c in range(min(8, len(sizes))):  # warm the allocator
+        plain(c)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for c in range(len(sizes)):
+        plain(c)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    same_idx = bool(torch.equal(pidx_d, idx_d))
+    same_nrm = bool(torch.equal(pnrm_d, nrm_d))
+
+    value = N / (ms_step * 1e-3)
+    plain_pps = N / t_plain
+    ab = N * (3 * 4 + 4 * k + 3 * 4)
+    achieved = ab / (tinfo.query_ms * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "points/s (kNN k=%d self query + normals k=%d per point, forest build "
+                "included)" % (k, k),
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": ("%d batched float32 clouds, sizes log-uniform [1e3, 2e5] (seed 3), "
+                         "%d points: forest build + kNN k=%d self query + normals k=%d, one "
+                         "hip-graph replay per step" % (len(sizes), N, k, k)),
+            "clouds": int(len(sizes)), "n_points": N, "k": k,
+            "leaf_size": best["leaf"], "query_block": best["block"],
+        },
+        "graph_ms_per_step": float(np.mean(times)),
+        "phases_ms_eager": {"build": tinfo.build_ms, "query": tinfo.query_ms},
+        "plain_path": {
+            "points_per_s": plain_pps,
+            "seconds": t_plain,
+            "what": "per cloud: pptk_tree_build + pptk_knn(all) + pptk_normals, same stream",
+            "results_identical_to_batched": {"knn": same_idx, "normals": same_nrm},
+        },
+        "speedup_vs_plain_path": value / plain_pps,
+        "batch": {"num_nodes": binfo.num_nodes, "depth": binfo.depth,
+                  "morton_bits": binfo.morton_bits, "kernel_launches": binfo.graph_nodes,
+                  "device_bytes": binfo.device_bytes,
+                  "normals_fp64_fallback_frac": binfo.num_fp64 / N},
+        "tuning": table,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": BATCH_KERNEL,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": measured_traffic(BATCH_KERNEL),
+            "algorithmic_bytes_per_launch": ab,
+        },
+    }
+    plan.free()
+    if not args.no_cpu:
+        log("bench: CPU baseline (batched)")
+        out["cpu_baseline"] = cpu_baseline_batched(clouds, k, min(16, os.cpu_count() or 1))
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="lidar", choices=["lidar", "batched"])
+    ap.add_argument("--clouds", type=int, default=4096)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
@@ -142,6 +346,8 @@ def main():
     ap.add_argument("--leaf", type=int, default=0)
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="batched: skip the plain-path comparison (profiling runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB mebibytes (counter calibration)")
     args = ap.parse_args()
@@ -161,6 +367,10 @@ def main():
         print("bench.py measures one GPU; --gpus %d is not supported (queries and trees "
               "shard by replication / by cloud, see DESIGN.md)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "batched":
+        bench_batched(args)
+        return
 
     import torch
 

@@ -232,6 +232,8 @@ typedef struct {
   int use_graph;       /* 1 = capture the run into a hip graph and replay it */
   int query_block;     /* threads per block of the query kernel: 64, 128 or
                           256 (0 = 256) */
+  int timing;          /* 1 = run eagerly (no graph) and time the build and
+                          query phases with hip events (pptk_batch_info) */
 } pptk_batch_params;
 
 typedef struct {
@@ -245,6 +247,9 @@ typedef struct {
   int graph_nodes;       /* kernel launches per run */
   int64_t device_bytes;  /* device memory held by the plan */
   int64_t num_fp64;      /* rows that took the FP64 eigensolve (last run) */
+  double build_ms;       /* last timed run: forest build (all kernels before
+                            the query kernel) */
+  double query_ms;       /* last timed run: fused kNN + normals kernel */
 } pptk_batch_info;
 
 /* Plan for clouds of the given sizes (host array of num_clouds counts,

@@ -62,7 +62,8 @@ class TreeInfo(ctypes.Structure):
 class BatchParams(ctypes.Structure):
     _fields_ = [("k", ctypes.c_int), ("normals_k", ctypes.c_int), ("dmax", ctypes.c_double),
                 ("max_leaf_size", ctypes.c_int), ("precision", ctypes.c_int),
-                ("use_graph", ctypes.c_int), ("query_block", ctypes.c_int)]
+                ("use_graph", ctypes.c_int), ("query_block", ctypes.c_int),
+                ("timing", ctypes.c_int)]
 
 
 class BatchInfo(ctypes.Structure):
@@ -70,7 +71,8 @@ class BatchInfo(ctypes.Structure):
                 ("num_valid", ctypes.c_int64), ("num_nodes", ctypes.c_int64),
                 ("depth", ctypes.c_int), ("leaf_size", ctypes.c_int),
                 ("morton_bits", ctypes.c_int), ("graph_nodes", ctypes.c_int),
-                ("device_bytes", ctypes.c_int64), ("num_fp64", ctypes.c_int64)]
+                ("device_bytes", ctypes.c_int64), ("num_fp64", ctypes.c_int64),
+                ("build_ms", ctypes.c_double), ("query_ms", ctypes.c_double)]
 
 
 _lib = None
@@ -303,7 +305,7 @@ class Batch:
     device buffer allocated up front, runs on device pointers."""
 
     def __init__(self, sizes, *, dim=3, dtype=np.float32, k=16, normals_k=16, dmax=np.inf,
-                 leaf=0, precision=PREC_AUTO, use_graph=True, query_block=0):
+                 leaf=0, precision=PREC_AUTO, use_graph=True, query_block=0, timing=False):
         sz = np.ascontiguousarray(np.asarray(sizes, np.int64))
         self.sizes = sz
         self.offsets = np.concatenate([[0], np.cumsum(sz)]).astype(np.int64)
@@ -312,7 +314,7 @@ class Batch:
         self.k = int(k)
         self.normals_k = int(normals_k)
         prm = BatchParams(self.k, self.normals_k, float(dmax), int(leaf), int(precision),
-                          1 if use_graph else 0, int(query_block))
+                          1 if use_graph else 0, int(query_block), 1 if timing else 0)
         h = ctypes.c_void_p()
         self.handle = None
         check(lib().pptk_batch_create(_p(sz), sz.shape[0], self.dim, dtype_code(self.dtype),

@@ -56,6 +56,9 @@ struct pptk_batch {
   int prec = PPTK_PREC_MIXED;
   bool use_graph = false;
   int qblock = 256;
+  bool timing = false;
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  double build_ms = 0.0, query_ms = 0.0;
   int launches = 0;
   pk::DevBuf coff;    // int64[C+1] cloud input offsets
   pk::DevBuf tiles;   // TileRec[ntiles]
@@ -557,6 +560,7 @@ void enqueue_run(pptk_batch* b, const D* pts, int32_t* out_idx, int32_t* out_cnt
   int launches = 0;
   auto lc = [&]() { PK_LAUNCH_CHECK(); ++launches; };
   BatchState* st = b->state.as<BatchState>();
+  if (b->timing) PK_HIP(hipEventRecord(b->ev[0], s));
   reset_kernel<<<1, kBlock, 0, s>>>(b->lev_s.as<int32_t>(), b->lev_e.as<int32_t>(),
                                     b->done.as<uint32_t>(), b->nlev, st);
   lc();
@@ -602,6 +606,7 @@ void enqueue_run(pptk_batch* b, const D* pts, int32_t* out_idx, int32_t* out_cnt
         b->leafof.as<int32_t>(), b->lev_s.as<int32_t>(), b->lev_e.as<int32_t>(), l);
     lc();
   }
+  if (b->timing) PK_HIP(hipEventRecord(b->ev[1], s));
   if (b->N > 0 && (b->k > 0 || b->nk > 0)) {
     TreeView<D, DIM> tv;
     tv.pts = b->rec.as<const PointRec<D, DIM>>();
@@ -653,6 +658,7 @@ void enqueue_run(pptk_batch* b, const D* pts, int32_t* out_idx, int32_t* out_cnt
     }
     lc();
   }
+  if (b->timing) PK_HIP(hipEventRecord(b->ev[2], s));
   b->launches = launches;
 }
 
@@ -743,8 +749,13 @@ pptk_status api_batch_create(const int64_t* sizes, int64_t C, int dim, pptk_dtyp
     b->prec = prm->precision == PPTK_PREC_AUTO
                   ? (dtype == PPTK_F64 ? PPTK_PREC_FP64 : PPTK_PREC_MIXED)
                   : prm->precision;
-    b->use_graph = prm->use_graph != 0;
+    b->timing = prm->timing != 0;
+    b->use_graph = prm->use_graph != 0 && !b->timing;
     b->qblock = qb;
+    if (b->timing) {
+      ensure_device();
+      for (auto& e : b->ev) PK_HIP(hipEventCreate(&e));
+    }
 
     std::vector<TileRec> tiles;
     std::vector<int32_t> tbeg((size_t)C + 1, 0);
@@ -856,6 +867,13 @@ pptk_status api_batch_run(pptk_batch* bp, const void* pts, int32_t* out_idx, int
     b->nn = hs.counter;
     b->nfb = (int64_t)hs.nfb;
     b->ran = true;
+    if (b->timing) {
+      float a = 0.f, q = 0.f;
+      PK_HIP(hipEventElapsedTime(&a, b->ev[0], b->ev[1]));
+      PK_HIP(hipEventElapsedTime(&q, b->ev[1], b->ev[2]));
+      b->build_ms = a;
+      b->query_ms = q;
+    }
   });
 }
 
@@ -871,6 +889,8 @@ pptk_status api_batch_get_info(const pptk_batch* bp, pptk_batch_info* out) {
     out->morton_bits = b->mb;
     out->graph_nodes = b->launches;
     out->num_fp64 = b->nfb;
+    out->build_ms = b->build_ms;
+    out->query_ms = b->query_ms;
     out->depth = 0;
     if (b->ran) {
       std::vector<int32_t> ls(b->nlev + 1), le(b->nlev + 1);
@@ -899,6 +919,8 @@ void api_batch_free(pptk_batch* b) {
     if (!g_batches.erase(b)) return;
   }
   drop_graph(b);
+  for (auto& e : b->ev)
+    if (e) hipEventDestroy(e);
   b->magic = 0;
   delete b;
 }

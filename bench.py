@@ -199,7 +199,7 @@ def bench_batched(args):
     torch.cuda.synchronize()
     log("bench: autotuning on %d clouds (%d points)" % (len(sel), sample_d.shape[0]))
     if args.leaf:
-        best, table = {"leaf": args.leaf, "block": 256, "ms": None}, []
+        best, table = {"leaf": args.leaf, "block": args.qblock, "ms": None}, []
     else:
         best, table = tune.tune_batch(sample_d, sizes[sel], k=k, normals_k=k, stream=stream,
                                       log=log, use_cache=False)
@@ -345,6 +345,8 @@ def main():
     ap.add_argument("--dmax", type=float, default=0.1)
     ap.add_argument("--leaf", type=int, default=0)
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--qblock", type=int, default=64,
+                    help="batched: query block width when --leaf fixes the bucket size")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true",
                     help="batched: skip the plain-path comparison (profiling runs)")

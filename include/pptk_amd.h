@@ -67,7 +67,7 @@ typedef struct pptk_tree pptk_tree;
 
 /* Build parameters. Replaces pointkd::BuildParams (pptk/kdtree/src/kdtree.h:29-42).
  * max_leaf_size: points per leaf bucket; <= 0 picks the tuned default,
- *   otherwise clamped to [1, 64]. The bucket size changes speed only, never
+ *   otherwise clamped to [1, 256]. The bucket size changes speed only, never
  *   results.
  * num_proc, empty_split: accepted for signature parity and ignored (the GPU
  *   tree uses tight per-node boxes, which trim empty space without
@@ -227,7 +227,7 @@ typedef struct {
   int normals_k;       /* normals neighbourhood size incl. the point, 0..33
                           (0 = no normals); dim must be 3 */
   double dmax;         /* distance bound of both (INFINITY = none) */
-  int max_leaf_size;   /* bucket size 1..64, 0 = tuned default */
+  int max_leaf_size;   /* bucket size 1..256, 0 = tuned default */
   int precision;       /* pptk_precision for the normals */
   int use_graph;       /* 1 = capture the run into a hip graph and replay it */
   int query_block;     /* threads per block of the query kernel: 64, 128 or

@@ -735,7 +735,7 @@ pptk_status api_batch_create(const int64_t* sizes, int64_t C, int dim, pptk_dtyp
     if (b->cap > (int64_t)INT32_MAX) fail(PPTK_E_VALUE, "too many points for int32 node ids");
     const int leaf = prm->max_leaf_size > 0 ? prm->max_leaf_size
                                             : default_leaf_size(dim, dtype == PPTK_F64, b->N);
-    b->leaf = std::min(64, std::max(1, leaf));
+    b->leaf = std::min(256, std::max(1, leaf));
     const int cbits = bits_for(C);  // room for ids 0..C (C = invalid rows)
     const int mbmax = dim == 2 ? 31 : dim == 3 ? 21 : 15;
     b->mb = std::min(mbmax, (63 - cbits) / dim);

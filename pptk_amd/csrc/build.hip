@@ -460,9 +460,12 @@ std::set<const void*> g_trees;
 
 }  // namespace
 
+// Measured on MI355X (10M-point lidar scene and 4096 batched clouds, leaf
+// sizes 8..192): kNN and normals are fastest at 64 points per leaf, where a
+// leaf scan is one 64-candidate chunk; r-near gains a few percent more at 96.
 int default_leaf_size(int dim, bool dbl, int64_t n) {
   (void)dim; (void)dbl; (void)n;
-  return 16;
+  return 64;
 }
 
 const pptk_tree* check_tree(const pptk_tree* t) {
@@ -495,7 +498,7 @@ pptk_status api_tree_build(const void* pts, int64_t n, int dim, pptk_dtype dtype
     t->n_input = n;
     int leaf = params && params->max_leaf_size > 0 ? params->max_leaf_size
                                                    : pk::default_leaf_size(dim, t->dist == PPTK_F64, n);
-    t->leaf = std::min(64, std::max(1, leaf));
+    t->leaf = std::min(256, std::max(1, leaf));
     pk::DevInput in;
     pk::stage_input(in, pts, (size_t)n * dim * pk::dtype_size(dtype), mem, s);
     pptk_tree* tp = t.get();

@@ -42,7 +42,7 @@ PyObject* Build(PyObject*, PyObject* args, PyObject* kwargs) {
     }
     const long v = PyLong_AsLong(idx.p);
     if (v == -1 && PyErr_Occurred()) return nullptr;
-    leaf = v < 1 ? 1 : (v > 64 ? 64 : (int)v);
+    leaf = v < 1 ? 1 : (v > 256 ? 256 : (int)v);
   }
   pyx::Array2D x;
   if (!pyx::extract_array2d(data, x)) {
@@ -251,7 +251,7 @@ const char kBuildDoc[] =
     "numprocs : int, optional\n"
     "    Accepted for compatibility; the tree is built on the GPU.\n"
     "maxleafsize : int, optional\n"
-    "    Points per leaf bucket (1..64). Default: tuned for the GPU.\n"
+    "    Points per leaf bucket (1..256). Default: tuned for the GPU.\n"
     "emptysplit : float, optional\n"
     "    Accepted for compatibility; node boxes are always tight.\n\n"
     "Returns\n-------\n"

@@ -122,7 +122,7 @@ def test_tree_structure_duplicates(gpu):
     from pptk_amd import capi
     pts = np.concatenate([np.zeros((5000, 3)), np.ones((3000, 3)),
                           np.random.default_rng(0).random((100, 3))]).astype(np.float32)
-    for leaf in (1, 7, 64):
+    for leaf in (1, 7, 64, 200):
         t = capi.Tree.build(pts, leaf=leaf)
         P, rev, fwd, box, nodes = t.export()
         check_tree_structure(P, box, nodes, leaf)
@@ -152,7 +152,7 @@ def test_knn_self_float32_k(gpu, k):
     assert np.array_equal(got[:, :k], want)
 
 
-@pytest.mark.parametrize("leaf", [1, 4, 10, 16, 32, 64])
+@pytest.mark.parametrize("leaf", [1, 4, 10, 16, 32, 64, 100, 256])
 def test_knn_leaf_size_does_not_change_results(gpu, leaf):
     rng = np.random.default_rng(11)
     pts = rng.normal(size=(20_000, 3))
@@ -160,6 +160,27 @@ def test_knn_leaf_size_does_not_change_results(gpu, leaf):
     got, _ = _rows(kd._query(kd._build(pts, maxleafsize=leaf), None, k=12))
     want, _ = O.OracleTree(pts).knn(None, 12)
     assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("leaf", [96, 256])
+def test_large_leaves_rnear_and_normals(gpu, leaf):
+    """Leaves above one wavefront (scanned in chunks of 64): r-near sets and
+    normals unchanged."""
+    from pptk_amd import capi
+    rng = np.random.default_rng(12)
+    pts = rng.normal(size=(30_000, 3)).astype(np.float32)
+    t = capi.Tree.build(pts, leaf=leaf)
+    t16 = capi.Tree.build(pts, leaf=16)
+    qi = rng.choice(len(pts), 3000, replace=False)
+    dm = float(np.float32(0.15))
+    off, idx = capi.rnear(t, qi, dm)
+    woff, widx = O.OracleTree(pts).rnear(qi, dm)
+    assert np.array_equal(off, woff)
+    for i in range(len(qi)):
+        assert np.array_equal(np.sort(idx[off[i]:off[i + 1]]), widx[woff[i]:woff[i + 1]])
+    a = capi.normals(t, 16, np.inf)
+    b = capi.normals(t16, 16, np.inf)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
 
 
 def test_knn_dmax_and_points_queries(gpu):

@@ -257,7 +257,7 @@ def make_queries(queries=None, *, dev_ptr=None, m=None, kind=None, dtype=None, m
     return q, keep
 
 
-def knn(tree, queries, k, dmax=np.inf, with_dist=False):
+def knn(tree, queries, k, dmax=np.inf, with_dist=False, stream=None):
     """Host-memory kNN: (idx (m,k) int32 -1 padded, counts (m,) [, dist])."""
     q, keep = make_queries(queries)
     m = tree.info.num_valid if q.kind == Q_ALL else q.m
@@ -265,26 +265,30 @@ def knn(tree, queries, k, dmax=np.inf, with_dist=False):
     cnt = np.empty(m, np.int32)
     dist = np.empty((m, k), tree.dist_dtype) if with_dist else None
     check(lib().pptk_knn(ctypes.c_void_p(tree.handle), ctypes.byref(q), int(k), float(dmax),
-                         _p(idx), _p(cnt), _p(dist), MEM_HOST, None))
+                         _p(idx), _p(cnt), _p(dist), MEM_HOST, stream))
     return (idx, cnt, dist) if with_dist else (idx, cnt)
 
 
-def rnear(tree, queries, dmax):
-    """Host-memory r-near: (offsets (m+1,) int64, idx int32)."""
+def rnear(tree, queries, dmax, stream=None, max_total=None):
+    """Host-memory r-near: (offsets (m+1,) int64, idx int32). With max_total,
+    raises MemoryError before the fill when the lists hold more entries."""
     q, keep = make_queries(queries)
     m = tree.info.num_valid if q.kind == Q_ALL else q.m
     off = np.empty(m + 1, np.int64)
     total = ctypes.c_int64()
     check(lib().pptk_rnear_count(ctypes.c_void_p(tree.handle), ctypes.byref(q), float(dmax),
-                                 _p(off), ctypes.byref(total), MEM_HOST, None))
+                                 _p(off), ctypes.byref(total), MEM_HOST, stream))
+    if max_total is not None and total.value > max_total:
+        raise MemoryError("r-near lists hold %d entries, more than the %d allowed"
+                          % (total.value, max_total))
     idx = np.empty(int(total.value), np.int32)
     check(lib().pptk_rnear_fill(ctypes.c_void_p(tree.handle), ctypes.byref(q), float(dmax),
-                                _p(off), _p(idx), MEM_HOST, None))
+                                _p(off), _p(idx), MEM_HOST, stream))
     return off, idx
 
 
 def normals(tree, k, r, rows=None, num_eigen=1, precision=PREC_AUTO, want_evals=True,
-            want_sizes=True):
+            want_sizes=True, stream=None):
     """Host-memory normals: (evecs, evals, sizes, num_fp64)."""
     inf = tree.info
     dt = np.float64 if inf.dtype == F64 else np.float32
@@ -296,7 +300,7 @@ def normals(tree, k, r, rows=None, num_eigen=1, precision=PREC_AUTO, want_evals=
     nfb = ctypes.c_int64()
     check(lib().pptk_normals(ctypes.c_void_p(tree.handle), int(k), float(r), _p(rr),
                              0 if rr is None else rr.shape[0], int(num_eigen), int(precision),
-                             _p(ev), _p(ew), _p(sz), ctypes.byref(nfb), MEM_HOST, None))
+                             _p(ev), _p(ew), _p(sz), ctypes.byref(nfb), MEM_HOST, stream))
     return ev, ew, sz, int(nfb.value)
 
 

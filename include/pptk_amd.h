@@ -152,6 +152,19 @@ pptk_status pptk_tree_get_info(const pptk_tree* tree, pptk_tree_info* out);
 pptk_status pptk_tree_export(const pptk_tree* tree, void* points, int32_t* reverse,
                              int32_t* forward, void* boxes, int32_t* nodes, pptk_mem mem);
 
+/* Persistent tree format: the whole tree as one flat little-endian byte
+ * buffer (header, coordinates, row maps, nodes, boxes, leaf map), restorable
+ * without a rebuild. Stands where pptk's unused flat node array does:
+ * SmallNode (small_node.h:13-45) from KdTree::SmallNodes()
+ * (kdtree-impl.h:605-627, 863-871). serialize: buf NULL returns the size in
+ * *out_size; otherwise writes it (capacity >= size). deserialize validates
+ * every index on the host before any device upload and fails with
+ * PPTK_E_VALUE on a damaged buffer. */
+pptk_status pptk_tree_serialize(const pptk_tree* tree, void* buf, int64_t capacity,
+                                int64_t* out_size);
+pptk_status pptk_tree_deserialize(const void* buf, int64_t size, void* stream,
+                                  pptk_tree** out);
+
 /* k nearest neighbours with distance < dmax. Replaces
  * KdTree::KNearestNeighbors / KNearestNeighborsSelf batch calls
  * (kdtree-impl.h:873-976) behind kdtree._query(k>0).

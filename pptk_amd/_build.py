@@ -22,7 +22,7 @@ OBJ = os.path.join(ROOT, "_build", "obj")
 LIBDIR = os.path.join(PKG, "lib")
 LIB = os.path.join(LIBDIR, "libpptk_amd.so")
 
-HIP_SOURCES = ["runtime.hip", "sort.hip", "build.hip", "query.hip", "normals.hip", "batch.hip"]
+HIP_SOURCES = ["runtime.hip", "sort.hip", "build.hip", "query.hip", "normals.hip", "batch.hip", "persist.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [

@@ -37,6 +37,7 @@ EXPORTED = [
     "pptk_tree_check", "pptk_tree_get_info", "pptk_tree_export", "pptk_knn", "pptk_rnear_count",
     "pptk_rnear_fill", "pptk_normals", "pptk_sort_pairs_u64", "pptk_exclusive_scan_i32",
     "pptk_batch_create", "pptk_batch_run", "pptk_batch_get_info", "pptk_batch_free",
+    "pptk_tree_serialize", "pptk_tree_deserialize",
 ]
 
 
@@ -120,6 +121,10 @@ def lib():
     L.pptk_batch_get_info.restype = ci
     L.pptk_batch_get_info.argtypes = [vp, ctypes.POINTER(BatchInfo)]
     L.pptk_batch_free.argtypes = [vp]
+    L.pptk_tree_serialize.restype = ci
+    L.pptk_tree_serialize.argtypes = [vp, vp, i64, ctypes.POINTER(i64)]
+    L.pptk_tree_deserialize.restype = ci
+    L.pptk_tree_deserialize.argtypes = [vp, i64, vp, ctypes.POINTER(vp)]
     _lib = L
     return L
 
@@ -206,6 +211,32 @@ class Tree:
     @property
     def dist_dtype(self):
         return np.float64 if self.info.dist_dtype == F64 else np.float32
+
+    def serialize(self):
+        """The tree in the persistent flat format (bytes)."""
+        n = ctypes.c_int64()
+        check(lib().pptk_tree_serialize(ctypes.c_void_p(self.handle), None, 0, ctypes.byref(n)))
+        buf = np.empty(n.value, np.uint8)
+        check(lib().pptk_tree_serialize(ctypes.c_void_p(self.handle), _p(buf), n.value,
+                                        ctypes.byref(n)))
+        return buf.tobytes()
+
+    @classmethod
+    def deserialize(cls, data, stream=None):
+        """A device tree from serialize()'s bytes (validated before upload)."""
+        buf = np.frombuffer(data, np.uint8)
+        h = ctypes.c_void_p()
+        check(lib().pptk_tree_deserialize(_p(buf), buf.size, stream, ctypes.byref(h)))
+        return cls(h.value)
+
+    def save(self, path):
+        with open(path, "wb") as f:
+            f.write(self.serialize())
+
+    @classmethod
+    def load(cls, path, stream=None):
+        with open(path, "rb") as f:
+            return cls.deserialize(f.read(), stream=stream)
 
     def export(self):
         """Host copies: points (tree order), reverse, forward, boxes (nn, 2, dim),

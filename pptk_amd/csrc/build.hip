@@ -468,6 +468,11 @@ int default_leaf_size(int dim, bool dbl, int64_t n) {
   return 64;
 }
 
+void register_tree(pptk_tree* t) {
+  std::lock_guard<std::mutex> lk(g_trees_mu);
+  g_trees.insert(t);
+}
+
 const pptk_tree* check_tree(const pptk_tree* t) {
   if (!t) fail(PPTK_E_TYPE, "null tree");
   std::lock_guard<std::mutex> lk(g_trees_mu);

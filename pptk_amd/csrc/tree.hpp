@@ -31,6 +31,10 @@ namespace pk {
 
 const pptk_tree* check_tree(const pptk_tree* t);
 
+// Adds a tree made outside pptk_tree_build (a deserialised one) to the set
+// of live handles.
+void register_tree(pptk_tree* t);
+
 template <typename D, int DIM>
 TreeView<D, DIM> tree_view(const pptk_tree* t) {
   TreeView<D, DIM> v;

@@ -223,6 +223,61 @@ class Server:
             while self._queue or self._busy:
                 self._cv.wait()
 
+    def checkpoint(self, directory):
+        """Writes every resident tree (persistent tree format) and an index
+        to `directory`; returns the number of trees written. Cloud ids must
+        be strings or integers."""
+        import json
+        import os
+        os.makedirs(directory, exist_ok=True)
+        with self._cv:
+            items = [(cid, e) for cid, e in self._trees.items() if e.tree is not None]
+            for _, e in items:
+                e.users += 1  # pinned while written
+        index = []
+        try:
+            for i, (cid, e) in enumerate(items):
+                if not isinstance(cid, (str, int)):
+                    raise TypeError("cloud id %r cannot be checkpointed" % (cid,))
+                name = "tree%06d.pptktree" % i
+                with open(os.path.join(directory, name), "wb") as f:
+                    f.write(self.backend.serialize(e.tree))
+                index.append({"cloud": cid, "file": name, "points": e.n})
+        finally:
+            with self._cv:
+                for _, e in items:
+                    e.users -= 1
+                self._cv.notify_all()
+        with open(os.path.join(directory, "index.json"), "w") as f:
+            json.dump(index, f)
+        return len(index)
+
+    def restore(self, directory):
+        """Loads the trees of checkpoint(directory) as resident trees (those
+        that fit the budget, evicting idle ones); returns how many."""
+        import json
+        import os
+        with open(os.path.join(directory, "index.json")) as f:
+            index = json.load(f)
+        loaded = 0
+        for item in index:
+            with open(os.path.join(directory, item["file"]), "rb") as f:
+                blob = f.read()
+            tree = self.backend.deserialize(blob, 0)
+            nbytes = int(self.backend.tree_bytes(tree))
+            with self._cv:
+                if item["cloud"] in self._trees or not self._evict_for(nbytes):
+                    self.backend.free(tree)
+                    continue
+                e = _Entry()
+                e.tree, e.bytes, e.n = tree, nbytes, int(item["points"])
+                e.last_use = self.clock()
+                e.ready.set()
+                self._trees[item["cloud"]] = e
+                self._resident += nbytes
+                loaded += 1
+        return loaded
+
     def memory(self):
         with self._cv:
             return {"reserved": self._reserved, "resident": self._resident,
@@ -423,6 +478,12 @@ class GpuBackend:
 
     def free(self, tree):
         tree.free()
+
+    def serialize(self, tree):
+        return tree.serialize()
+
+    def deserialize(self, blob, slot):
+        return self.capi.Tree.deserialize(blob, stream=self._s(slot))
 
     def run(self, req, tree, slot, rnear_cap):
         c = self.capi

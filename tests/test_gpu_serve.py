@@ -106,3 +106,21 @@ def test_served_results_match_oracle(gpu):
     gap = (few[:, 1] - few[:, 0]) / np.maximum(few[:, 2], 1e-300)
     ang = O.angle_deg(nrm, fev[:, 0, :])
     assert ang[gap >= 1e-3].max() < 0.5
+
+
+def test_checkpoint_restore_on_device(gpu, tmp_path):
+    from pptk_amd import serve
+    clouds = _clouds()[:3]
+    s = serve.Server(serve.GpuBackend(2), slots=2)
+    first = [s.submit(serve.Request("knn", "c%d" % i, c, k=16)).result(timeout=120)
+             for i, c in enumerate(clouds)]
+    assert s.checkpoint(str(tmp_path)) == 3
+    s.close()
+    s2 = serve.Server(serve.GpuBackend(2), slots=2)
+    assert s2.restore(str(tmp_path)) == 3
+    again = [s2.submit(serve.Request("knn", "c%d" % i, None, k=16)).result(timeout=120)
+             for i in range(3)]
+    assert not any(t.built for t in s2.completed)
+    s2.close()
+    for a, b in zip(first, again):
+        assert _same(a, b)

@@ -66,6 +66,12 @@ class FakeBackend:
     def free(self, tree):
         tree.pts = None
 
+    def serialize(self, tree):
+        return np.ascontiguousarray(tree.pts, np.float32).tobytes()
+
+    def deserialize(self, blob, slot):
+        return FakeTree(np.frombuffer(blob, np.float32).reshape(-1, 3).copy())
+
     def run(self, req, tree, slot, rnear_cap):
         self._enter()
         try:
@@ -290,3 +296,23 @@ def test_open_loop_stats():
     assert st["errors"] == 0 and st["requests"] == 40
     assert st["p50_ms"] <= st["p99_ms"]
     s.close()
+
+
+def test_checkpoint_and_restore_resident_trees(tmp_path):
+    rng = np.random.default_rng(10)
+    cl = clouds(rng, 4)
+    be = FakeBackend(delay=0.0)
+    s = make_server(be, slots=2)
+    for i, c in enumerate(cl):
+        s.submit(serve.Request("knn", "c%d" % i, c, k=3)).result(timeout=30)
+    assert s.checkpoint(str(tmp_path)) == 4
+    s.close()
+    be2 = FakeBackend(delay=0.0)
+    s2 = make_server(be2, slots=2)
+    assert s2.restore(str(tmp_path)) == 4
+    # follow-ups without points run on the restored trees, nothing rebuilt
+    for i, c in enumerate(cl):
+        r = s2.submit(serve.Request("normals", "c%d" % i, None, k=4))
+        assert same(r.result(timeout=30), brute(r.req, c))
+    assert be2.builds == 0
+    s2.close()

@@ -83,6 +83,17 @@ def measured_traffic(kernel):
         return None
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(pts, qpts, k, dmax, gpu_knn_rows=None):
     """Time pptk's k-d tree (or the oracle port) on a bounded sample, and
     compare its kNN rows with the GPU's rows for the same queries."""
@@ -114,6 +125,16 @@ def cpu_baseline(pts, qpts, k, dmax, gpu_knn_rows=None):
     port.normals(k, np.inf, rows=np.arange(mn), nthreads=cores)
     t_nrm = time.perf_counter() - t0
     del port
+    # one host thread, on a smaller sample
+    m1 = min(n, 100_000)
+    tree1 = O.RefTree(pts) if kind == "reference" else O.OracleTree(pts)
+    t0 = time.perf_counter()
+    if kind == "reference":
+        tree1.knn(None, k, nthreads=1, m_self=m1)
+    else:
+        tree1.knn(np.arange(m1), k, nthreads=1)
+    knn_rate_1t = m1 / (time.perf_counter() - t0)
+    del tree1
     knn_rate = mq / t_knn
     nrm_rate = mn / t_nrm
     rn_rate = mr / t_rn
@@ -134,6 +155,9 @@ def cpu_baseline(pts, qpts, k, dmax, gpu_knn_rows=None):
         "rnear_qps": rn_rate,
         "knn_rows_identical_to_cpu": same,
         "knn_rows_compared": mq,
+        "knn_qps_1thread": knn_rate_1t,
+        "cpu_model": cpu_model(),
+        "nproc": os.cpu_count(),
     }
 
 

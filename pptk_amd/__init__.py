@@ -10,8 +10,12 @@ Both run on hand-written gfx950 HIP kernels in ``lib/libpptk_amd.so``; there
 is no CPU fallback. Importing raises ImportError when the native modules have
 not been built (``python -m pptk_amd._build``).
 """
-from .kdtree import kdtree
-from .processing.estimate_normals.estimate_normals import estimate_normals
+from ._runtime import preload_hip_runtime
+
+preload_hip_runtime()
+
+from .kdtree import kdtree  # noqa: E402
+from .processing.estimate_normals.estimate_normals import estimate_normals  # noqa: E402
 
 __all__ = ["kdtree", "estimate_normals"]
 

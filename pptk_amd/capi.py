@@ -84,6 +84,8 @@ def lib():
     global _lib
     if _lib is not None:
         return _lib
+    from ._runtime import preload_hip_runtime
+    preload_hip_runtime()
     L = ctypes.CDLL(LIB_PATH)
     vp, i64, ci, cd = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_double
     L.pptk_last_error.restype = ctypes.c_char_p

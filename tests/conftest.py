@@ -21,9 +21,9 @@ def gpu_count():
 
 
 def _init_torch_first():
-    """Tests that hand torch device buffers to the library need torch's HIP
-    runtime initialised before the library's own: torch cannot start its
-    runtime once the library's has claimed the device in this process."""
+    """Starts torch's GPU context before the tests (they hand torch device
+    buffers and streams to the library; pptk_amd loads torch's HIP runtime,
+    so both share one runtime)."""
     try:
         import torch
     except Exception:

@@ -287,6 +287,127 @@ __global__ __launch_bounds__(kBlock) void rnear_kernel(TreeView<D, DIM> tv, int 
   }
 }
 
+// r-near on a single-root tree, one traversal per wave (count: FILL = false,
+// fill: FILL = true). The 64 queries of a wave are neighbours on the Morton
+// curve, so their balls overlap: the wave walks the tree once, in
+// left-to-right order, and visits a node when any lane's ball reaches it.
+// Every lane classifies the node against its own ball:
+//   * inside (box upper bound < r*r, RNearNeighborsHelper,
+//     kdtree-impl.h:527-529): the lane takes the whole range [a, b) and skips
+//     the node's subtree (it remembers b; nodes starting before b are done);
+//   * straddling: at a leaf, the leaf's points are loaded once (one per lane)
+//     and tested against each straddling lane's query in turn, the hits
+//     ranked with a ballot; at an inner node the wave descends.
+// Each lane's output is in tree order, exactly as rnear_kernel's.
+template <typename D, int DIM, bool FILL>
+__global__ __launch_bounds__(kBlock) void rnear_packet_kernel(TreeView<D, DIM> tv, int mode,
+                                                              const int32_t* __restrict__ qlist,
+                                                              const D* __restrict__ qpts,
+                                                              const int32_t* __restrict__ order,
+                                                              int64_t m, int64_t nrows, D r2,
+                                                              int32_t* __restrict__ counts,
+                                                              const int64_t* __restrict__ offsets,
+                                                              int32_t* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const uint64_t lt_mask = (1ull << lane) - 1ull;
+  D q[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) q[d] = D(0);
+  int32_t excl = -1;
+  int64_t row = 0;
+  bool ok = false, has_row = false;
+  if (t < m)
+    has_row = resolve_query<D, DIM>(tv, mode, qlist, qpts, order, t, nrows, q, excl, row, ok);
+  const bool active = has_row && ok;
+  int64_t base = 0, cap = 0;
+  if (FILL && active) {
+    base = offsets[row];
+    cap = offsets[row + 1] - base;
+  }
+  int32_t cnt = 0;                  // outputs so far for this lane's query
+  int32_t done_end = INT32_MIN;     // nodes starting before this are taken
+  int32_t node = __ballot(active) ? 0 : -1;
+  while (node >= 0) {
+    node = __builtin_amdgcn_readfirstlane(node);
+    const NodeRec nr = tv.node[node];
+    const BoxRec<D, DIM> bx = tv.box[node];
+    const bool hit = active && nr.a >= done_end && box_min_dist2<D, DIM>(bx, q) < r2;
+    bool descend = false;
+    if (__ballot(hit)) {
+      const bool inside = hit && box_max_dist2<D, DIM>(bx, q) < r2;
+      const bool strad = hit && !inside;
+      const uint64_t insm = __ballot(inside);
+      const uint64_t strm = __ballot(strad);
+      const int32_t start = cnt;    // this lane's output position for the node
+      if (inside) {
+        const bool ex = excl >= nr.a && excl < nr.b;
+        cnt += (nr.b - nr.a) - (ex ? 1 : 0);
+        done_end = nr.b;
+      }
+      if constexpr (FILL) {
+        if (insm) {
+          // copy the range once per chunk, into every inside lane's list
+          for (int32_t s0 = nr.a; s0 < nr.b; s0 += 64) {
+            const int32_t p = s0 + lane;
+            const bool valid = p < nr.b;
+            const int32_t og = valid ? tv.orig[p] : 0;
+            uint64_t lm = insm;
+            while (lm) {
+              const int L = __builtin_ctzll(lm);
+              lm &= lm - 1ull;
+              const int32_t exL = __shfl(excl, L);
+              const int32_t stL = __shfl(start, L);
+              const int64_t bL = __shfl(base, L);
+              const int64_t cL = __shfl(cap, L);
+              if (valid && p != exL) {
+                const bool ex_in = exL >= nr.a && exL < nr.b;
+                const int64_t o = (int64_t)stL + (p - nr.a) - ((ex_in && p > exL) ? 1 : 0);
+                if (o < cL) out[bL + o] = og;
+              }
+            }
+          }
+        }
+      }
+      if (strm) {
+        if (nr.child < 0) {
+          for (int32_t s0 = nr.a; s0 < nr.b; s0 += 64) {
+            const int32_t p = s0 + lane;
+            const bool valid = p < nr.b;
+            PointRec<D, DIM> rec{};
+            if (valid) rec = tv.pts[p];
+            uint64_t sm = strm;
+            while (sm) {
+              const int L = __builtin_ctzll(sm);
+              sm &= sm - 1ull;
+              D qL[DIM];
+#pragma unroll
+              for (int d = 0; d < DIM; ++d) qL[d] = __shfl(q[d], L);
+              const int32_t exL = __shfl(excl, L);
+              const bool pass = valid && p != exL && dist2<D, DIM>(qL, rec.c) < r2;
+              const uint64_t hits = __ballot(pass);
+              if constexpr (FILL) {
+                const int32_t cntL = __shfl(cnt, L);
+                const int64_t bL = __shfl(base, L);
+                const int64_t cL = __shfl(cap, L);
+                const int64_t o = (int64_t)cntL + __popcll(hits & lt_mask);
+                if (pass && o < cL) out[bL + o] = rec.orig;
+              }
+              if (lane == L) cnt += __popcll(hits);
+            }
+          }
+        } else {
+          descend = true;
+        }
+      }
+    }
+    node = descend ? nr.child : next_in_order<D, DIM>(tv, node);
+  }
+  if constexpr (!FILL) {
+    if (has_row) counts[row] = ok ? cnt : 0;
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void fill_kernel(T* __restrict__ p, int64_t n, T v) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -510,10 +631,14 @@ void rnear_count_run(const pptk_tree* t, const pptk_queries* qs, double dmax, in
   if (nrows > 0) PK_HIP(hipMemsetAsync(counts.get(), 0, (size_t)nrows * 4, s));
   const TreeView<D, DIM> tv = tree_view<D, DIM>(t);
   if (pq.m > 0) {
-    rnear_kernel<D, DIM, false><<<grid_of(pq.m), kBlock, 0, s>>>(
-        tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.order, pq.m, nrows, r2,
-        counts.as<int32_t>(),
-        nullptr, nullptr);
+    if (tv.nroots == 1)
+      rnear_packet_kernel<D, DIM, false><<<grid_of(pq.m), kBlock, 0, s>>>(
+          tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.order, pq.m, nrows, r2,
+          counts.as<int32_t>(), nullptr, nullptr);
+    else
+      rnear_kernel<D, DIM, false><<<grid_of(pq.m), kBlock, 0, s>>>(
+          tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.order, pq.m, nrows, r2,
+          counts.as<int32_t>(), nullptr, nullptr);
     PK_LAUNCH_CHECK();
   }
   int64_t* doff = (int64_t*)oo.dptr;
@@ -551,9 +676,14 @@ void rnear_fill_run(const pptk_tree* t, const pptk_queries* qs, double dmax, con
   if (total > 0 && !out_idx) fail(PPTK_E_VALUE, "null output");
   const TreeView<D, DIM> tv = tree_view<D, DIM>(t);
   if (pq.m > 0 && total > 0) {
-    rnear_kernel<D, DIM, true><<<grid_of(pq.m), kBlock, 0, s>>>(
-        tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.order, pq.m, nrows, r2, nullptr,
-        (const int64_t*)doff.ptr, (int32_t*)oi.dptr);
+    if (tv.nroots == 1)
+      rnear_packet_kernel<D, DIM, true><<<grid_of(pq.m), kBlock, 0, s>>>(
+          tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.order, pq.m, nrows, r2, nullptr,
+          (const int64_t*)doff.ptr, (int32_t*)oi.dptr);
+    else
+      rnear_kernel<D, DIM, true><<<grid_of(pq.m), kBlock, 0, s>>>(
+          tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.order, pq.m, nrows, r2, nullptr,
+          (const int64_t*)doff.ptr, (int32_t*)oi.dptr);
     PK_LAUNCH_CHECK();
   }
   oi.finish(s);

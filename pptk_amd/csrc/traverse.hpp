@@ -176,21 +176,56 @@ struct HeapK {
   }
 };
 
+// Largest value below x (x >= 0): d < x  <=>  d <= below(x) for every
+// non-NaN d, so the radius test and the k-th bound test become one compare.
+__device__ __forceinline__ float below(float x) {
+  return x > 0.f ? __uint_as_float(__float_as_uint(x) - 1u) : -1.f;
+}
+__device__ __forceinline__ double below(double x) {
+  return x > 0.0 ? __longlong_as_double(__double_as_longlong(x) - 1ll) : -1.0;
+}
+
 template <typename D, int DIM, typename State>
 __device__ __forceinline__ void knn_scan_leaf(const TreeView<D, DIM>& tv, int32_t a, int32_t b,
                                               const D* q, D r2, int32_t excl, State& st) {
-  // Leaves hold at most 64 points unless a capped build stopped splitting;
-  // scan in chunks of 64 so the candidate mask always fits.
+  // Leaves hold at most 256 points; scan in chunks of 64 so the candidate
+  // mask fits one 64-bit word. Each chunk first marks the points that pass
+  // d <= min(k-th bound, below(r*r)) -- exactly d < r*r and d <= k-th --
+  // four loads in flight at a time (eight cost too many registers, and the
+  // lost occupancy outweighs the saved instructions), then inserts the
+  // marked ones.
+  const D r2b = below(r2);
   for (int32_t c0 = a; c0 < b; c0 += 64) {
-    const int32_t c1 = b - c0 > 64 ? c0 + 64 : b;
+    const int32_t cnt = b - c0 > 64 ? 64 : b - c0;
     const D kb = st.bound();
+    const D thr = kb < r2b ? kb : r2b;
+    const PointRec<D, DIM>* base = tv.pts + c0;
     uint64_t mask = 0;
-    for (int32_t p = c0; p < c1; ++p) {
-      const PointRec<D, DIM> r = tv.pts[p];
-      const D dd = dist2<D, DIM>(q, r.c);
-      const bool pass = (dd < r2) && (dd <= kb) && (p != excl);
-      mask |= (uint64_t)pass << (p - c0);
+    for (int32_t j = 0; j < cnt; j += 4) {
+      uint32_t grp = 0;
+      if (j + 4 <= cnt) {
+        // full group: four independent loads at fixed offsets
+        const PointRec<D, DIM>* g = base + j;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const D dd = dist2<D, DIM>(q, g[u].c);
+          grp |= dd <= thr ? (1u << u) : 0u;
+        }
+      } else {
+        // tail: indices clamped to the last point, extra bits cleared
+        const int32_t last = cnt - 1;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int32_t i = j + u < last ? j + u : last;
+          const D dd = dist2<D, DIM>(q, base[i].c);
+          grp |= dd <= thr ? (1u << u) : 0u;
+        }
+        grp &= (1u << (cnt - j)) - 1u;
+      }
+      mask |= (uint64_t)grp << j;
     }
+    const int32_t ex = excl - c0;
+    if (ex >= 0 && ex < cnt) mask &= ~(1ull << ex);
     while (mask) {
       const int s = __builtin_ctzll(mask);
       mask &= mask - 1ull;

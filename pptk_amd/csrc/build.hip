@@ -23,10 +23,12 @@
 //   K3 node boxes      bottom-up, one launch per level: tight box of each
 //                      leaf's points, union of the children's boxes above.
 #include <algorithm>
+#include <cstring>
 #include <mutex>
 #include <set>
 #include <vector>
 
+#include "levels.hpp"
 #include "morton.hpp"
 #include "sort.hpp"
 #include "tree.hpp"
@@ -36,7 +38,7 @@ namespace pk {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kPer = 16;                 // consecutive points per thread
+constexpr int kPer = 16;                 // rows of kBlock points per tile
 constexpr int kTile = kBlock * kPer;     // 4096
 
 template <typename T>
@@ -68,13 +70,14 @@ __global__ __launch_bounds__(kBlock) void ingest_reduce_kernel(const T* __restri
                                                                uint32_t* __restrict__ blk_cnt,
                                                                D* __restrict__ blk_lo,
                                                                D* __restrict__ blk_hi) {
-  const int64_t first = (int64_t)blockIdx.x * kTile + (int64_t)threadIdx.x * kPer;
+  // tile rows of kBlock consecutive points: lane-consecutive loads
+  const int64_t first = (int64_t)blockIdx.x * kTile + threadIdx.x;
   uint32_t cnt = 0;
   D lo[DIM], hi[DIM];
 #pragma unroll
   for (int d = 0; d < DIM; ++d) { lo[d] = pos_inf<D>(); hi[d] = -pos_inf<D>(); }
   for (int e = 0; e < kPer; ++e) {
-    const int64_t i = first + e;
+    const int64_t i = first + (int64_t)e * kBlock;
     if (i >= n) break;
     T v[DIM];
     bool ok = true;
@@ -209,49 +212,62 @@ __global__ __launch_bounds__(kBlock) void ingest_finalize_kernel(const uint32_t*
   }
 }
 
+// Stable compaction of a tile's valid rows and their Morton codes. Rows are
+// read lane-consecutively (row e of the tile = points [e*kBlock, (e+1)*kBlock)
+// of it); a row's valid points are ranked with a wave ballot, and the
+// per-(row, wave) counts are scanned in (row, wave) order, which is index
+// order, so the compaction keeps the input order.
 template <typename T, typename D, int DIM>
 __global__ __launch_bounds__(kBlock) void morton_kernel(const T* __restrict__ pts, int64_t n,
                                                         const uint32_t* __restrict__ blk_off,
                                                         const BuildInfo<D>* __restrict__ info_p,
                                                         uint64_t* __restrict__ keys,
                                                         uint32_t* __restrict__ vals) {
+  constexpr int kWaves = kBlock / 64;
+  static_assert(kPer * kWaves == 64, "one wave scans the (row, wave) counts");
+  __shared__ uint32_t s_off[kPer * kWaves];
   const BuildInfo<D> info = *info_p;
-  const int64_t first = (int64_t)blockIdx.x * kTile + (int64_t)threadIdx.x * kPer;
-  uint32_t flags = 0, cnt = 0;
-  for (int e = 0; e < kPer; ++e) {
-    const int64_t i = first + e;
-    if (i >= n) break;
-    bool ok = true;
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) ok = ok && elem_finite(pts[i * DIM + d]);
-    if (ok) { flags |= 1u << e; ++cnt; }
-  }
-  // block exclusive scan of per-thread counts
-  __shared__ uint32_t s_wsum[kBlock / 64];
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
-  uint32_t incl = cnt;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const uint32_t y = __shfl_up(incl, o);
-    if (lane >= o) incl += y;
-  }
-  if (lane == 63) s_wsum[w] = incl;
-  __syncthreads();
-  uint32_t off = 0;
-#pragma unroll
-  for (int i = 0; i < kBlock / 64; ++i)
-    if (i < w) off += s_wsum[i];
-  uint32_t pos = blk_off[blockIdx.x] + off + incl - cnt;
+  const uint64_t lt_mask = (1ull << lane) - 1ull;
+  const int64_t first = (int64_t)blockIdx.x * kTile + threadIdx.x;
+  uint32_t flags = 0;
   for (int e = 0; e < kPer; ++e) {
-    if (!((flags >> e) & 1u)) continue;
-    const int64_t i = first + e;
+    const int64_t i = first + (int64_t)e * kBlock;
+    bool ok = i < n;
+    if (ok) {
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) ok = ok && elem_finite(pts[i * DIM + d]);
+    }
+    flags |= (uint32_t)ok << e;
+    const uint64_t b = __ballot(ok);
+    if (lane == 0) s_off[e * kWaves + w] = (uint32_t)__popcll(b);
+  }
+  __syncthreads();
+  if (w == 0) {
+    // exclusive scan of the kPer * kWaves (= 64) counts, one per lane
+    const uint32_t c = s_off[lane];
+    uint32_t incl = c;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(incl, o);
+      if (lane >= o) incl += y;
+    }
+    s_off[lane] = incl - c;
+  }
+  __syncthreads();
+  const uint32_t tile0 = blk_off[blockIdx.x];
+  for (int e = 0; e < kPer; ++e) {
+    const bool ok = (flags >> e) & 1u;
+    const uint64_t b = __ballot(ok);
+    if (!ok) continue;
+    const int64_t i = first + (int64_t)e * kBlock;
+    const uint32_t pos = tile0 + s_off[e * kWaves + w] + (uint32_t)__popcll(b & lt_mask);
     D c[DIM];
 #pragma unroll
     for (int d = 0; d < DIM; ++d) c[d] = (D)pts[i * DIM + d];
     keys[pos] = morton_code<D, DIM>(c, info);
     vals[pos] = (uint32_t)i;
-    ++pos;
   }
 }
 
@@ -271,82 +287,6 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(const T* __restrict__ pt
   rec[p] = r;
   orig[p] = (int32_t)i;
   fwd[i] = (int32_t)p;
-}
-
-// One level of the top-down split. Nodes [s, e) are this level; a node with
-// more than `leaf` points gets two children appended at *counter.
-__global__ __launch_bounds__(kBlock) void split_level_kernel(const uint64_t* __restrict__ keys,
-                                                             NodeRec* __restrict__ nodes,
-                                                             int64_t s, int64_t e, int leaf,
-                                                             int* __restrict__ counter) {
-  const int64_t i = s + (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (i >= e) return;
-  const int32_t a = nodes[i].a, b = nodes[i].b;
-  if (b - a <= leaf) {
-    nodes[i].child = -1;
-    return;
-  }
-  const uint64_t x = keys[a] ^ keys[b - 1];
-  int32_t split;
-  if (x == 0ull) {
-    split = a + (b - a) / 2;
-  } else {
-    // Codes in [a, b) share every bit above `bit`; keys[a] has it clear and
-    // keys[b-1] set, so the first code with it set lies in (a, b-1].
-    const int bit = 63 - __clzll((long long)x);
-    int32_t lo = a + 1, hi = b - 1;
-    while (lo < hi) {
-      const int32_t mid = lo + ((hi - lo) >> 1);
-      if ((keys[mid] >> bit) & 1ull) hi = mid;
-      else lo = mid + 1;
-    }
-    split = lo;
-  }
-  const int c = atomicAdd(counter, 2);
-  NodeRec l, r;
-  l.a = a; l.b = split; l.child = -1; l.parent = (int32_t)i;
-  r.a = split; r.b = b; r.child = -1; r.parent = (int32_t)i;
-  nodes[c] = l;
-  nodes[c + 1] = r;
-  nodes[i].child = c;
-}
-
-// One level of the bottom-up box pass over nodes [s, e).
-template <typename D, int DIM>
-__global__ __launch_bounds__(kBlock) void node_box_kernel(const PointRec<D, DIM>* __restrict__ rec,
-                                                          const NodeRec* __restrict__ nodes,
-                                                          BoxRec<D, DIM>* __restrict__ box,
-                                                          int32_t* __restrict__ leafof,
-                                                          int64_t s, int64_t e) {
-  const int64_t i = s + (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (i >= e) return;
-  const NodeRec nr = nodes[i];
-  BoxRec<D, DIM> b;
-#pragma unroll
-  for (int d = 0; d < PadDim<DIM>::value; ++d) {
-    b.lo[d] = d < DIM ? pos_inf<D>() : D(0);
-    b.hi[d] = d < DIM ? -pos_inf<D>() : D(0);
-  }
-  if (nr.child < 0) {
-    for (int32_t p = nr.a; p < nr.b; ++p) {
-      leafof[p] = (int32_t)i;
-      const PointRec<D, DIM> r = rec[p];
-#pragma unroll
-      for (int d = 0; d < DIM; ++d) {
-        b.lo[d] = r.c[d] < b.lo[d] ? r.c[d] : b.lo[d];
-        b.hi[d] = r.c[d] > b.hi[d] ? r.c[d] : b.hi[d];
-      }
-    }
-  } else {
-    const BoxRec<D, DIM> x = box[nr.child];
-    const BoxRec<D, DIM> y = box[nr.child + 1];
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) {
-      b.lo[d] = x.lo[d] < y.lo[d] ? x.lo[d] : y.lo[d];
-      b.hi[d] = x.hi[d] > y.hi[d] ? x.hi[d] : y.hi[d];
-    }
-  }
-  box[i] = b;
 }
 
 unsigned grid_for(int64_t n, int block) {
@@ -394,14 +334,30 @@ void build_impl(pptk_tree* t, const T* dpts, int64_t n, hipStream_t s) {
   // A binary tree whose leaves are non-empty has at most 2*nv - 1 nodes.
   const int64_t cap = std::max<int64_t>(1, 2 * nv - 1);
   DevBuf nodes((size_t)cap * sizeof(NodeRec));
-  DevBuf counter(sizeof(int));
-  std::vector<std::pair<int64_t, int64_t>> levels;
+  // Level table: every split launch reads its range from device memory and
+  // publishes the next level, so the host reads it back only once per
+  // kLevelBatch levels to see whether the tree is complete. Depth is bounded
+  // by the key bits plus 31 halvings of equal keys.
+  constexpr int kLevelBatch = 8;
+  const int max_levels = DIM * MortonBits<DIM>::value + 33;
+  DevBuf lev((size_t)(2 * (max_levels + 1)) * 4);
+  DevBuf done((size_t)max_levels * 4);
+  DevBuf state(sizeof(LevelState));
+  int32_t* lev_s = lev.as<int32_t>();
+  int32_t* lev_e = lev_s + (max_levels + 1);
   {
     NodeRec root;
     root.a = 0; root.b = (int32_t)nv; root.child = -1; root.parent = -1;
-    const int one = 1;
+    LevelState st0;
+    std::memset(&st0, 0, sizeof(st0));
+    st0.counter = 1;
+    const int32_t l0[2] = {0, 1};
     PK_HIP(hipMemcpyAsync(nodes.get(), &root, sizeof(root), hipMemcpyHostToDevice, s));
-    PK_HIP(hipMemcpyAsync(counter.get(), &one, sizeof(int), hipMemcpyHostToDevice, s));
+    PK_HIP(hipMemcpyAsync(state.get(), &st0, sizeof(st0), hipMemcpyHostToDevice, s));
+    PK_HIP(hipMemsetAsync(lev.get(), 0, lev.bytes(), s));
+    PK_HIP(hipMemsetAsync(done.get(), 0, done.bytes(), s));
+    PK_HIP(hipMemcpyAsync(lev_s, &l0[0], 4, hipMemcpyHostToDevice, s));
+    PK_HIP(hipMemcpyAsync(lev_e, &l0[1], 4, hipMemcpyHostToDevice, s));
   }
   DevBuf keys, vals;
   if (nv > 0) {
@@ -421,35 +377,51 @@ void build_impl(pptk_tree* t, const T* dpts, int64_t n, hipStream_t s) {
     PK_LAUNCH_CHECK();
   }
   // K3: split level by level until a level creates no children.
-  int64_t ls = 0, le = 1;
-  while (ls < le) {
-    levels.emplace_back(ls, le);
-    if (nv > 0) {
-      split_level_kernel<<<grid_for(le - ls, kBlock), kBlock, 0, s>>>(
-          keys.as<uint64_t>(), nodes.as<NodeRec>(), ls, le, (int)B, counter.as<int>());
-      PK_LAUNCH_CHECK();
+  const unsigned split_grid =
+      (unsigned)std::min<int64_t>(1024, std::max<int64_t>(1, ceil_div(nv, (int64_t)B * kBlock)));
+  int nlev = 1;  // levels [0, nlev) hold nodes
+  if (nv > 0) {
+    for (int l0 = 0;; l0 += kLevelBatch) {
+      const int l1 = std::min(l0 + kLevelBatch, max_levels);
+      for (int l = l0; l < l1; ++l) {
+        split_level_dev_kernel<<<split_grid, kLevelBlock, 0, s>>>(
+            keys.as<uint64_t>(), nodes.as<NodeRec>(), lev_s, lev_e, done.as<uint32_t>(), l,
+            max_levels, (int)B, cap, state.as<LevelState>());
+        PK_LAUNCH_CHECK();
+      }
+      int32_t next[2] = {0, 0};
+      PK_HIP(hipMemcpyAsync(&next[0], lev_s + l1, 4, hipMemcpyDeviceToHost, s));
+      PK_HIP(hipMemcpyAsync(&next[1], lev_e + l1, 4, hipMemcpyDeviceToHost, s));
+      PK_HIP(hipStreamSynchronize(s));
+      if (next[1] <= next[0] || l1 >= max_levels) break;
     }
-    int cnt = 0;
-    PK_HIP(hipMemcpyAsync(&cnt, counter.get(), sizeof(int), hipMemcpyDeviceToHost, s));
-    PK_HIP(hipStreamSynchronize(s));
-    if (cnt > cap) fail(PPTK_E_RUNTIME, "tree build: node count %d exceeds bound %lld", cnt,
-                        (long long)cap);
-    ls = le;
-    le = cnt;
   }
-  const int64_t nn = le;
+  LevelState st;
+  std::vector<int32_t> hs((size_t)max_levels + 1), he((size_t)max_levels + 1);
+  PK_HIP(hipMemcpyAsync(&st, state.get(), sizeof(st), hipMemcpyDeviceToHost, s));
+  PK_HIP(hipMemcpyAsync(hs.data(), lev_s, hs.size() * 4, hipMemcpyDeviceToHost, s));
+  PK_HIP(hipMemcpyAsync(he.data(), lev_e, he.size() * 4, hipMemcpyDeviceToHost, s));
+  PK_HIP(hipStreamSynchronize(s));
+  if (st.overflow) fail(PPTK_E_RUNTIME, "tree build exceeded its node or level bound");
+  nlev = 0;
+  for (int l = 0; l <= max_levels; ++l)
+    if (he[l] > hs[l]) nlev = l + 1;
+  nlev = std::max(nlev, 1);
+  const int64_t nn = st.counter;
   t->nn = nn;
-  t->depth = (int)levels.size();
+  t->depth = nlev;
   t->nleaves = (nn + 1) / 2;
   t->node.reset((size_t)nn * sizeof(NodeRec));
   PK_HIP(hipMemcpyAsync(t->node.get(), nodes.get(), (size_t)nn * sizeof(NodeRec),
                         hipMemcpyDeviceToDevice, s));
   t->box.reset((size_t)nn * sizeof(BoxRec<D, DIM>));
   t->leafof.reset((size_t)std::max<int64_t>(nv, 1) * 4);
-  for (auto it = levels.rbegin(); it != levels.rend(); ++it) {
-    node_box_kernel<D, DIM><<<grid_for(it->second - it->first, kBlock), kBlock, 0, s>>>(
+  for (int l = nlev - 1; l >= 0; --l) {
+    const int64_t width = he[l] - hs[l];
+    const unsigned g = (unsigned)std::min<int64_t>(1024, std::max<int64_t>(1, ceil_div(width, kBlock)));
+    box_level_dev_kernel<D, DIM><<<g, kLevelBlock, 0, s>>>(
         t->pts.as<PointRec<D, DIM>>(), t->node.as<NodeRec>(), t->box.as<BoxRec<D, DIM>>(),
-        t->leafof.as<int32_t>(), it->first, it->second);
+        t->leafof.as<int32_t>(), lev_s, lev_e, l);
     PK_LAUNCH_CHECK();
   }
   PK_HIP(hipStreamSynchronize(s));  // temporaries return to the pool

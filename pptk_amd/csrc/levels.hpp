@@ -1,0 +1,123 @@
+// Device-driven level kernels of the top-down tree build, shared by the
+// single-tree build and the batched forest build.
+//
+// A split launch reads its node range [lev_s[l], lev_e[l]) from device
+// memory, splits every node of the range that holds more than `leaf` points
+// (at the highest key bit where its first and last keys differ, found by
+// binary search, or at the middle when all its keys are equal), appends the
+// children in adjacent pairs at the shared counter, and its last block to
+// finish publishes level l + 1. The box launches then run bottom-up over the
+// same level table. No host read is needed between levels.
+#pragma once
+#include "common.hpp"
+
+namespace pk {
+
+constexpr int kLevelBlock = 256;
+
+struct LevelState {
+  int32_t counter;    // next free node id
+  int32_t overflow;   // node capacity or level bound exceeded
+  int32_t pad[2];
+  unsigned long long nfb;  // batch: rows that took the FP64 eigensolve
+};
+
+// Split every node of level l (the same rule as the single-tree build: the
+// highest differing key bit, or the middle when all keys are equal). The last
+// block to finish publishes level l+1 = the pairs this launch appended.
+static __global__ __launch_bounds__(kLevelBlock) void split_level_dev_kernel(const uint64_t* __restrict__ keys,
+                                                                 NodeRec* __restrict__ nodes,
+                                                                 int32_t* __restrict__ lev_s,
+                                                                 int32_t* __restrict__ lev_e,
+                                                                 uint32_t* __restrict__ done,
+                                                                 int l, int nlev, int leaf,
+                                                                 int64_t cap,
+                                                                 LevelState* __restrict__ st) {
+  const int32_t s = lev_s[l], e = lev_e[l];
+  const int64_t stride = (int64_t)gridDim.x * kLevelBlock;
+  for (int64_t i = s + (int64_t)blockIdx.x * kLevelBlock + threadIdx.x; i < e; i += stride) {
+    const int32_t a = nodes[i].a, b = nodes[i].b;
+    if (b - a <= leaf) continue;  // child stays -1
+    const uint64_t x = keys[a] ^ keys[b - 1];
+    int32_t split;
+    if (x == 0ull) {
+      split = a + (b - a) / 2;
+    } else {
+      const int bit = 63 - __clzll((long long)x);
+      int32_t lo = a + 1, hi = b - 1;
+      while (lo < hi) {
+        const int32_t mid = lo + ((hi - lo) >> 1);
+        if ((keys[mid] >> bit) & 1ull) hi = mid;
+        else lo = mid + 1;
+      }
+      split = lo;
+    }
+    const int c = atomicAdd(&st->counter, 2);
+    if ((int64_t)c + 2 > cap || l + 1 >= nlev) {
+      st->overflow = 1;
+      continue;
+    }
+    NodeRec lc, rc;
+    lc.a = a; lc.b = split; lc.child = -1; lc.parent = (int32_t)i;
+    rc.a = split; rc.b = b; rc.child = -1; rc.parent = (int32_t)i;
+    nodes[c] = lc;
+    nodes[c + 1] = rc;
+    nodes[i].child = c;
+  }
+  __shared__ int last;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    last = atomicAdd(&done[l], 1u) == gridDim.x - 1u;
+  }
+  __syncthreads();
+  if (last && threadIdx.x == 0) {
+    __threadfence();
+    const int32_t c = atomicAdd(&st->counter, 0);
+    lev_s[l + 1] = e;
+    lev_e[l + 1] = c < (int32_t)cap ? c : (int32_t)cap;
+  }
+}
+
+template <typename D, int DIM>
+static __global__ __launch_bounds__(kLevelBlock) void box_level_dev_kernel(const PointRec<D, DIM>* __restrict__ rec,
+                                                               const NodeRec* __restrict__ nodes,
+                                                               BoxRec<D, DIM>* __restrict__ box,
+                                                               int32_t* __restrict__ leafof,
+                                                               const int32_t* __restrict__ lev_s,
+                                                               const int32_t* __restrict__ lev_e,
+                                                               int l) {
+  const int32_t s = lev_s[l], e = lev_e[l];
+  const int64_t stride = (int64_t)gridDim.x * kLevelBlock;
+  for (int64_t i = s + (int64_t)blockIdx.x * kLevelBlock + threadIdx.x; i < e; i += stride) {
+    const NodeRec nr = nodes[i];
+    BoxRec<D, DIM> b;
+#pragma unroll
+    for (int d = 0; d < PadDim<DIM>::value; ++d) {
+      b.lo[d] = d < DIM ? pos_inf<D>() : D(0);
+      b.hi[d] = d < DIM ? -pos_inf<D>() : D(0);
+    }
+    if (nr.child < 0) {
+      for (int32_t p = nr.a; p < nr.b; ++p) {
+        leafof[p] = (int32_t)i;
+        const PointRec<D, DIM> r = rec[p];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+          b.lo[d] = r.c[d] < b.lo[d] ? r.c[d] : b.lo[d];
+          b.hi[d] = r.c[d] > b.hi[d] ? r.c[d] : b.hi[d];
+        }
+      }
+    } else {
+      const BoxRec<D, DIM> x = box[nr.child];
+      const BoxRec<D, DIM> y = box[nr.child + 1];
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        b.lo[d] = x.lo[d] < y.lo[d] ? x.lo[d] : y.lo[d];
+        b.hi[d] = x.hi[d] > y.hi[d] ? x.hi[d] : y.hi[d];
+      }
+    }
+    box[i] = b;
+  }
+}
+
+}  // namespace pk

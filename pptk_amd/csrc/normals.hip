@@ -82,7 +82,7 @@ __global__ __launch_bounds__(kBlock) void normals_knn_kernel(TreeView<D, 3> tv,
   if (row < 0) return;
   TopK<D, K, true> st;
   st.init(k);
-  if (ok) knn_search_from_leaf<D, 3>(tv, pos, q, r2, -1, st);
+  if (ok) knn_search_from_leaf<D, 3>(tv, pos, q, r2, -1, k, st);
   int n = 0;
 #pragma unroll
   for (int s = 0; s < K; ++s) n += (s >= K - k && st.dist(s) < pos_inf<D>()) ? 1 : 0;
@@ -125,7 +125,7 @@ __global__ __launch_bounds__(kBlock) void normals_heap_kernel(TreeView<D, 3> tv,
   const int64_t w0 = (t - t0) * k;
   HeapK<D> st;
   st.init(ws_d + w0, ws_o + w0, ws_p + w0, k);
-  if (ok) knn_search_from_leaf<D, 3>(tv, pos, q, r2, -1, st);
+  if (ok) knn_search_from_leaf<D, 3>(tv, pos, q, r2, -1, k, st);
   const int n = st.size;
   bool fb = false;
   if (n == 0) {

@@ -41,7 +41,7 @@ __global__ __launch_bounds__(kBlock) void knn_kernel(TreeView<D, DIM> tv, int mo
   TopK<D, K, false> st;
   st.init(k);
   if (ok) {
-    if (excl >= 0) knn_search_from_leaf<D, DIM>(tv, excl, q, r2, excl, st);
+    if (excl >= 0) knn_search_from_leaf<D, DIM>(tv, excl, q, r2, excl, k, st);
     else knn_search<D, DIM>(tv, q, r2, excl, st);
   }
   int32_t c = 0;
@@ -82,7 +82,7 @@ __global__ __launch_bounds__(kBlock) void knn_large_kernel(TreeView<D, DIM> tv, 
   HeapK<D> st;
   st.init(hd, ho, nullptr, k);
   if (ok) {
-    if (excl >= 0) knn_search_from_leaf<D, DIM>(tv, excl, q, r2, excl, st);
+    if (excl >= 0) knn_search_from_leaf<D, DIM>(tv, excl, q, r2, excl, k, st);
     else knn_search<D, DIM>(tv, q, r2, excl, st);
   }
   st.finish();

@@ -187,7 +187,8 @@ __device__ __forceinline__ double below(double x) {
 
 template <typename D, int DIM, typename State>
 __device__ __forceinline__ void knn_scan_leaf(const TreeView<D, DIM>& tv, int32_t a, int32_t b,
-                                              const D* q, D r2, int32_t excl, State& st) {
+                                              const D* q, D r2, int32_t excl, State& st,
+                                              D cap = pos_inf<D>()) {
   // Leaves hold at most 256 points; scan in chunks of 64 so the candidate
   // mask fits one 64-bit word. Each chunk first marks the points that pass
   // d <= min(k-th bound, below(r*r)) -- exactly d < r*r and d <= k-th --
@@ -198,7 +199,8 @@ __device__ __forceinline__ void knn_scan_leaf(const TreeView<D, DIM>& tv, int32_
   for (int32_t c0 = a; c0 < b; c0 += 64) {
     const int32_t cnt = b - c0 > 64 ? 64 : b - c0;
     const D kb = st.bound();
-    const D thr = kb < r2b ? kb : r2b;
+    D thr = kb < r2b ? kb : r2b;
+    thr = cap < thr ? cap : thr;
     const PointRec<D, DIM>* base = tv.pts + c0;
     uint64_t mask = 0;
     for (int32_t j = 0; j < cnt; j += 4) {
@@ -305,12 +307,34 @@ __device__ void knn_search(const TreeView<D, DIM>& tv, const D* q, D r2, int32_t
 // while that child's box can still hold a better neighbour. Same result as
 // knn_search (every box that can hold a neighbour is still visited); the
 // nearest candidates are simply found first.
+//
+// ksel > 0 is the number of neighbours sought. Before the first scan, the
+// ksel points nearest to `pos` in tree order (skipping `excl`) give an upper
+// bound on the ksel-th distance -- the ksel-th smallest distance is at most
+// the largest distance of any ksel distinct candidates -- so the first leaf
+// marks only points that can still make the list.
 template <typename D, int DIM, typename State>
 __device__ void knn_search_from_leaf(const TreeView<D, DIM>& tv, int32_t pos, const D* q, D r2,
-                                     int32_t excl, State& st) {
+                                     int32_t excl, int ksel, State& st) {
   int32_t node = tv.leafof[pos];
   NodeRec cur = tv.node[node];
-  knn_scan_leaf<D, DIM>(tv, cur.a, cur.b, q, r2, excl, st);
+  D cap = pos_inf<D>();
+  const int32_t nsel = ksel + (excl >= 0 ? 1 : 0);
+  if (ksel > 0 && cur.b - cur.a >= nsel) {
+    // window of nsel positions inside the leaf that contains pos
+    int32_t s0 = pos - nsel / 2;
+    s0 = s0 < cur.a ? cur.a : s0;
+    s0 = s0 > cur.b - nsel ? cur.b - nsel : s0;
+    D mx = D(0);
+    for (int32_t j = 0; j < nsel; ++j) {
+      const int32_t p = s0 + j;
+      if (p == excl) continue;
+      const D dd = dist2<D, DIM>(q, tv.pts[p].c);
+      mx = dd > mx ? dd : mx;
+    }
+    cap = mx;
+  }
+  knn_scan_leaf<D, DIM>(tv, cur.a, cur.b, q, r2, excl, st, cap);
   while (cur.parent >= 0) {
     const int32_t sib = (node & 1) ? node + 1 : node - 1;
     const D ms = box_min_dist2<D, DIM>(tv.box[sib], q);

@@ -299,6 +299,22 @@ __global__ __launch_bounds__(kBlock) void rnear_kernel(TreeView<D, DIM> tv, int 
 //     and tested against each straddling lane's query in turn, the hits
 //     ranked with a ballot; at an inner node the wave descends.
 // Each lane's output is in tree order, exactly as rnear_kernel's.
+// Value of lane L (wave-uniform) in a scalar register: v_readlane, no LDS.
+__device__ __forceinline__ int32_t rl_i32(int32_t x, int L) {
+  return __builtin_amdgcn_readlane(x, L);
+}
+__device__ __forceinline__ int64_t rl_i64(int64_t x, int L) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int32_t)(uint32_t)x, L);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int32_t)(uint32_t)((uint64_t)x >> 32), L);
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ float rl(float x, int L) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), L));
+}
+__device__ __forceinline__ double rl(double x, int L) {
+  return __longlong_as_double(rl_i64(__double_as_longlong(x), L));
+}
+
 template <typename D, int DIM, bool FILL>
 __global__ __launch_bounds__(kBlock) void rnear_packet_kernel(TreeView<D, DIM> tv, int mode,
                                                               const int32_t* __restrict__ qlist,
@@ -356,10 +372,10 @@ __global__ __launch_bounds__(kBlock) void rnear_packet_kernel(TreeView<D, DIM> t
             while (lm) {
               const int L = __builtin_ctzll(lm);
               lm &= lm - 1ull;
-              const int32_t exL = __shfl(excl, L);
-              const int32_t stL = __shfl(start, L);
-              const int64_t bL = __shfl(base, L);
-              const int64_t cL = __shfl(cap, L);
+              const int32_t exL = rl_i32(excl, L);
+              const int32_t stL = rl_i32(start, L);
+              const int64_t bL = rl_i64(base, L);
+              const int64_t cL = rl_i64(cap, L);
               if (valid && p != exL) {
                 const bool ex_in = exL >= nr.a && exL < nr.b;
                 const int64_t o = (int64_t)stL + (p - nr.a) - ((ex_in && p > exL) ? 1 : 0);
@@ -382,14 +398,14 @@ __global__ __launch_bounds__(kBlock) void rnear_packet_kernel(TreeView<D, DIM> t
               sm &= sm - 1ull;
               D qL[DIM];
 #pragma unroll
-              for (int d = 0; d < DIM; ++d) qL[d] = __shfl(q[d], L);
-              const int32_t exL = __shfl(excl, L);
+              for (int d = 0; d < DIM; ++d) qL[d] = rl(q[d], L);
+              const int32_t exL = rl_i32(excl, L);
               const bool pass = valid && p != exL && dist2<D, DIM>(qL, rec.c) < r2;
               const uint64_t hits = __ballot(pass);
               if constexpr (FILL) {
-                const int32_t cntL = __shfl(cnt, L);
-                const int64_t bL = __shfl(base, L);
-                const int64_t cL = __shfl(cap, L);
+                const int32_t cntL = rl_i32(cnt, L);
+                const int64_t bL = rl_i64(base, L);
+                const int64_t cL = rl_i64(cap, L);
                 const int64_t o = (int64_t)cntL + __popcll(hits & lt_mask);
                 if (pass && o < cL) out[bL + o] = rec.orig;
               }

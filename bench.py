@@ -457,11 +457,11 @@ def bench_served(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", default="lidar", choices=["lidar", "batched", "served"])
-    ap.add_argument("--rates", default="25,50,100,200,400",
+    ap.add_argument("--rates", default="400,800,1000,1200,1400,1600,2000",
                     help="served: offered request rates (requests/s)")
     ap.add_argument("--requests", type=int, default=200, help="served: requests per rate")
     ap.add_argument("--pool", type=int, default=40, help="served: distinct clouds")
-    ap.add_argument("--slots", type=int, default=8, help="served: concurrent requests")
+    ap.add_argument("--slots", type=int, default=16, help="served: concurrent requests")
     ap.add_argument("--outputs", default="device", choices=["device", "host"],
                     help="served: where results are returned")
     ap.add_argument("--clouds", type=int, default=4096)

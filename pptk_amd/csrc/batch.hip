@@ -69,7 +69,7 @@ struct pptk_batch {
   pk::DevBuf vcnt;    // uint32[C+1]: valid counts -> exclusive offsets, [C] = total
   pk::DevBuf keys, vals;
   pk::SortWorkspace ws;
-  pk::DevBuf rec, gorig, leafof, nodes, box;
+  pk::DevBuf rec, gorig, fwd, leafof, nodes, box;
   pk::DevBuf lev_s, lev_e, done;
   pk::DevBuf state;   // BatchState
   // hip graph of one run, keyed on the buffers it was captured with
@@ -314,12 +314,15 @@ __global__ __launch_bounds__(kBlock) void batch_gather_kernel(const D* __restric
                                                               const uint32_t* __restrict__ vstart,
                                                               int64_t C,
                                                               PointRec<D, DIM>* __restrict__ rec,
-                                                              int32_t* __restrict__ gorig) {
+                                                              int32_t* __restrict__ gorig,
+                                                              int32_t* __restrict__ fwd) {
   const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (p >= N) return;
   const uint32_t i = vals[p];
   gorig[p] = (int32_t)i;
-  if (p >= (int64_t)vstart[C]) return;
+  const bool valid = p < (int64_t)vstart[C];
+  fwd[i] = valid ? (int32_t)p : -1;
+  if (!valid) return;
   const int64_t c = (int64_t)(keys[p] >> shift);
   PointRec<D, DIM> r;
 #pragma unroll
@@ -380,7 +383,10 @@ __global__ __launch_bounds__(kBlock) void batch_query_kernel(TreeView<D, DIM> tv
   if (t >= N) return;
   const int64_t row = gorig[t];
   const bool valid = t < (int64_t)*vtotal;
-  TopK<D, K, NORM> st;
+  // Positions are not carried through the top-k insertions: the normals
+  // read each neighbour's tree position from the forward map (global row =
+  // this point's global row - its cloud-local index + the neighbour's).
+  TopK<D, K, false> st;
   st.init(kk);
   D q[DIM];
   int32_t self = -1;
@@ -426,7 +432,7 @@ __global__ __launch_bounds__(kBlock) void batch_query_kernel(TreeView<D, DIM> tv
               ++taken;
               pending = false;
             }
-            if (taken < nk) f(st.p[s]);
+            if (taken < nk) f(tv.fwd[row - self + st.idx(s)]);
             ++taken;
           }
         }
@@ -484,7 +490,7 @@ void enqueue_run(pptk_batch* b, const D* pts, int32_t* out_idx, int32_t* out_cnt
     batch_gather_kernel<D, DIM><<<grid_of(b->N, kBlock), kBlock, 0, s>>>(
         pts, b->keys.as<uint64_t>(), b->vals.as<uint32_t>(), b->N, b->shift,
         b->coff.as<int64_t>(), b->vcnt.as<uint32_t>(), b->C, b->rec.as<PointRec<D, DIM>>(),
-        b->gorig.as<int32_t>());
+        b->gorig.as<int32_t>(), b->fwd.as<int32_t>());
     lc();
   }
   roots_kernel<<<grid_of(b->R, kBlock), kBlock, 0, s>>>(b->vcnt.as<uint32_t>(), b->C, b->R,
@@ -509,7 +515,7 @@ void enqueue_run(pptk_batch* b, const D* pts, int32_t* out_idx, int32_t* out_cnt
     TreeView<D, DIM> tv;
     tv.pts = b->rec.as<const PointRec<D, DIM>>();
     tv.orig = b->gorig.as<const int32_t>();
-    tv.fwd = nullptr;
+    tv.fwd = b->fwd.as<const int32_t>();
     tv.box = b->box.as<const BoxRec<D, DIM>>();
     tv.node = b->nodes.as<const NodeRec>();
     tv.leafof = b->leafof.as<const int32_t>();
@@ -694,6 +700,7 @@ pptk_status api_batch_create(const int64_t* sizes, int64_t C, int dim, pptk_dtyp
     });
     b->rec.reset((size_t)N1 * rec_sz);
     b->gorig.reset((size_t)N1 * 4);
+    b->fwd.reset((size_t)N1 * 4);
     b->leafof.reset((size_t)N1 * 4);
     b->nodes.reset((size_t)b->cap * sizeof(NodeRec));
     b->box.reset((size_t)b->cap * box_sz);
@@ -804,7 +811,7 @@ pptk_status api_batch_get_info(const pptk_batch* bp, pptk_batch_info* out) {
                                   b->cinfo.bytes() + b->vcnt.bytes() + b->keys.bytes() +
                                   b->vals.bytes() + b->ws.kalt.bytes() + b->ws.valt.bytes() +
                                   b->ws.hist.bytes() + b->ws.offs.bytes() + b->ws.ssum.bytes() +
-                                  b->rec.bytes() + b->gorig.bytes() + b->leafof.bytes() +
+                                  b->rec.bytes() + b->gorig.bytes() + b->fwd.bytes() + b->leafof.bytes() +
                                   b->nodes.bytes() + b->box.bytes() + b->lev_s.bytes() +
                                   b->lev_e.bytes() + b->done.bytes() + b->state.bytes());
   });

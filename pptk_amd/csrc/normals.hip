@@ -80,7 +80,9 @@ __global__ __launch_bounds__(kBlock) void normals_knn_kernel(TreeView<D, 3> tv,
   bool ok;
   resolve_row<D>(tv, rows, t, q, row, pos, ok);
   if (row < 0) return;
-  TopK<D, K, true> st;
+  // Positions are not carried through the top-k insertions: after the
+  // search each neighbour's tree position is read from the forward map.
+  TopK<D, K, false> st;
   st.init(k);
   if (ok) knn_search_from_leaf<D, 3>(tv, pos, q, r2, -1, k, st);
   int n = 0;
@@ -93,7 +95,7 @@ __global__ __launch_bounds__(kBlock) void normals_knn_kernel(TreeView<D, 3> tv,
     auto each = [&](auto f) {
 #pragma unroll
       for (int s = 0; s < K; ++s)
-        if (s >= K - k && st.dist(s) < pos_inf<D>()) f(st.p[s]);
+        if (s >= K - k && st.dist(s) < pos_inf<D>()) f(tv.fwd[st.idx(s)]);
     };
     double cd[3][3];
     float cf[3][3];

@@ -7,8 +7,10 @@
 //   2. exclusive scan of the histogram (scan_* kernels below) gives every
 //      (digit, tile) its global output offset;
 //   3. radix_scatter_kernel ranks keys stably inside the tile with wave
-//      ballots (peer mask of equal digits) and per-wave LDS counters, then
-//      scatters.
+//      ballots (peer mask of equal digits) and per-wave LDS counters, stages
+//      the tile in LDS in digit order, and writes each digit's run of the
+//      tile contiguously (coalesced, instead of one scattered address per
+//      lane).
 // A tile is 4096 keys: 4 waves x 16 rows x 64 lanes. Wave w owns the 1024
 // consecutive keys [w*1024, w*1024+1024) of the tile and reads them row by
 // row, so (wave, row, lane) order equals input order and the sort is stable.
@@ -56,6 +58,10 @@ __global__ __launch_bounds__(kBlock) void radix_scatter_kernel(
     const uint32_t* __restrict__ offs, int tiles) {
   __shared__ uint32_t cnt[kWaves][kRadix];
   __shared__ uint32_t gbase[kRadix];
+  __shared__ uint32_t dstart[kRadix];
+  __shared__ uint32_t wsum[kWaves];
+  __shared__ uint64_t skey[kTile];
+  __shared__ uint32_t sval[kTile];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int w = tid >> 6;
@@ -98,6 +104,7 @@ __global__ __launch_bounds__(kBlock) void radix_scatter_kernel(
     rank[j] = c0 + before;
   }
   __syncthreads();
+  uint32_t total;
   {
     // Exclusive prefix over waves for digit `tid` (kBlock == kRadix).
     uint32_t run = 0;
@@ -107,17 +114,45 @@ __global__ __launch_bounds__(kBlock) void radix_scatter_kernel(
       cnt[ww][tid] = run;
       run += c;
     }
+    total = run;  // keys of digit `tid` in this tile
+  }
+  // Tile-local start of each digit: exclusive prefix of the totals.
+  {
+    uint32_t incl = total;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(incl, o);
+      if (lane >= o) incl += y;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    uint32_t off = 0;
+#pragma unroll
+    for (int ww = 0; ww < kWaves; ++ww)
+      if (ww < w) off += wsum[ww];
+    dstart[tid] = off + incl - total;
   }
   __syncthreads();
+  // Stage the tile in digit order, then write each digit's run contiguously.
 #pragma unroll
   for (int j = 0; j < kItems; ++j) {
     const int64_t idx = wbase + (int64_t)j * 64 + lane;
     if (idx < n) {
       const uint32_t d = (uint32_t)(key[j] >> shift) & 255u;
-      const uint32_t pos = gbase[d] + cnt[w][d] + rank[j];
-      kout[pos] = key[j];
-      vout[pos] = val[j];
+      const uint32_t lp = dstart[d] + cnt[w][d] + rank[j];
+      skey[lp] = key[j];
+      sval[lp] = val[j];
     }
+  }
+  __syncthreads();
+  const int64_t tbase = (int64_t)blockIdx.x * kTile;
+  const int32_t nt = (int32_t)(n - tbase < kTile ? n - tbase : kTile);
+  for (int32_t i = tid; i < nt; i += kBlock) {
+    const uint64_t k = skey[i];
+    const uint32_t d = (uint32_t)(k >> shift) & 255u;
+    const uint32_t pos = gbase[d] + (uint32_t)i - dstart[d];
+    kout[pos] = k;
+    vout[pos] = sval[i];
   }
 }
 

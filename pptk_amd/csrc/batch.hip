@@ -304,7 +304,18 @@ __global__ __launch_bounds__(kBlock) void batch_morton_kernel(const D* __restric
   }
 }
 
+// Blocks are dispatched round-robin over the 8 XCDs (each with its own L2).
+// Remap so that each XCD serves one contiguous range of logical blocks, i.e.
+// one spatially coherent range of tree positions.
+__device__ __forceinline__ int64_t xcd_block() {
+  const int64_t g = gridDim.x, p = blockIdx.x;
+  const int64_t g8 = g - (g & 7);
+  return p < g8 ? (p & 7) * (g8 >> 3) + (p >> 3) : p;
+}
+
 // Tree-order point records. Positions >= num_valid hold the invalid rows.
+// Each XCD serves a contiguous range of positions, i.e. of clouds, so the
+// rows it gathers stay within a few clouds' input and hit its L2.
 template <typename D, int DIM>
 __global__ __launch_bounds__(kBlock) void batch_gather_kernel(const D* __restrict__ pts,
                                                               const uint64_t* __restrict__ keys,
@@ -316,7 +327,7 @@ __global__ __launch_bounds__(kBlock) void batch_gather_kernel(const D* __restric
                                                               PointRec<D, DIM>* __restrict__ rec,
                                                               int32_t* __restrict__ gorig,
                                                               int32_t* __restrict__ fwd) {
-  const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t p = xcd_block() * kBlock + threadIdx.x;
   if (p >= N) return;
   const uint32_t i = vals[p];
   gorig[p] = (int32_t)i;
@@ -352,15 +363,6 @@ __global__ __launch_bounds__(kBlock) void roots_kernel(const uint32_t* __restric
     lev_s[0] = 0;
     lev_e[0] = (int32_t)R;
   }
-}
-
-// Blocks are dispatched round-robin over the 8 XCDs (each with its own L2).
-// Remap so that each XCD serves one contiguous range of logical blocks, i.e.
-// one spatially coherent range of tree positions.
-__device__ __forceinline__ int64_t xcd_block() {
-  const int64_t g = gridDim.x, p = blockIdx.x;
-  const int64_t g8 = g - (g & 7);
-  return p < g8 ? (p & 7) * (g8 >> 3) + (p >> 3) : p;
 }
 
 // Fused self-query kNN + PCA normals, one lane per tree position. The search

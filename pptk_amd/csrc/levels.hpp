@@ -98,7 +98,23 @@ static __global__ __launch_bounds__(kLevelBlock) void box_level_dev_kernel(const
       b.hi[d] = d < DIM ? -pos_inf<D>() : D(0);
     }
     if (nr.child < 0) {
-      for (int32_t p = nr.a; p < nr.b; ++p) {
+      // four independent loads in flight per step
+      int32_t p = nr.a;
+      for (; p + 4 <= nr.b; p += 4) {
+        PointRec<D, DIM> r[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) r[u] = rec[p + u];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          leafof[p + u] = (int32_t)i;
+#pragma unroll
+          for (int d = 0; d < DIM; ++d) {
+            b.lo[d] = r[u].c[d] < b.lo[d] ? r[u].c[d] : b.lo[d];
+            b.hi[d] = r[u].c[d] > b.hi[d] ? r[u].c[d] : b.hi[d];
+          }
+        }
+      }
+      for (; p < nr.b; ++p) {
         leafof[p] = (int32_t)i;
         const PointRec<D, DIM> r = rec[p];
 #pragma unroll

@@ -15,7 +15,7 @@ import os
 
 import numpy as np
 
-LEAVES = (8, 12, 16, 24, 32, 48, 64)
+LEAVES = (8, 12, 16, 24, 32, 48, 64, 96, 128)
 BLOCKS = (64, 128, 256)
 
 
@@ -25,16 +25,17 @@ def _cache_path():
                                        "tune.json"))
 
 
-def profile_key(sizes, dim, dtype, k, normals_k):
-    """Key of a workload: device, shape parameters and the size distribution
-    in octaves (clouds per power-of-two size class)."""
+def profile_key(sizes, dim, dtype, k, normals_k, leaves=LEAVES, blocks=BLOCKS):
+    """Key of a workload: device, shape parameters, the size distribution
+    in octaves (clouds per power-of-two size class) and the candidate set."""
     import torch
     dev = torch.cuda.get_device_name(0) if torch.cuda.is_available() else "cpu"
     sz = np.asarray(sizes, np.int64)
     hist = np.bincount(np.log2(np.maximum(sz, 1)).astype(int), minlength=32)
     frac = np.round(hist / max(len(sz), 1), 2)
-    return "%s|dim%d|%s|k%d|nk%d|%s" % (dev, dim, np.dtype(dtype).name, k, normals_k,
-                                        ",".join("%g" % f for f in frac))
+    return "%s|dim%d|%s|k%d|nk%d|%s|L%s|B%s" % (
+        dev, dim, np.dtype(dtype).name, k, normals_k, ",".join("%g" % f for f in frac),
+        ",".join(map(str, leaves)), ",".join(map(str, blocks)))
 
 
 def load_cache():
@@ -65,7 +66,7 @@ def tune_batch(pts_d, sizes, *, dim=3, dtype=np.float32, k=16, normals_k=16, dma
 
     from . import capi
 
-    key = profile_key(sizes, dim, dtype, k, normals_k)
+    key = profile_key(sizes, dim, dtype, k, normals_k, leaves, blocks)
     if use_cache:
         hit = load_cache().get(key)
         if hit is not None:

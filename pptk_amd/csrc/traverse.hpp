@@ -97,18 +97,47 @@ struct TopK<float, K, POS> {
     return __uint_as_float((uint32_t)(key[K - 1] >> 32));
   }
   __device__ __forceinline__ bool beats(float dd, int32_t oo) const { return make(dd, oo) < key[K - 1]; }
+
+  // Unsigned 64-bit min / max of two keys in one instruction. Every key read
+  // as an IEEE double is non-negative and finite: its top 32 bits are a
+  // squared distance's float bits (sign 0, at most +inf = 0x7f800000, so the
+  // double exponent stays below 0x7ff), and non-negative doubles order as
+  // their bit patterns. Denormal doubles (distances below 2^-126) are exact
+  // because f64 denormals are preserved. Written in asm because the
+  // builtins first canonicalise each operand (one more instruction each).
+  static __device__ __forceinline__ uint64_t kmin(uint64_t a, uint64_t b) {
+    uint64_t r;
+    asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+  }
+  static __device__ __forceinline__ uint64_t kmax(uint64_t a, uint64_t b) {
+    uint64_t r;
+    asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+  }
+
   __device__ __forceinline__ void insert(float dd, int32_t oo, int32_t pp) {
     const uint64_t kk = make(dd, oo);
+    if constexpr (POS) {
 #pragma unroll
-    for (int s = K - 1; s > 0; --s) {
-      const bool sh = kk < key[s - 1];
-      const bool here = kk < key[s];
-      key[s] = sh ? key[s - 1] : (here ? kk : key[s]);
-      if constexpr (POS) p[s] = sh ? p[s - 1] : (here ? pp : p[s]);
+      for (int s = K - 1; s > 0; --s) {
+        const bool sh = kk < key[s - 1];
+        const bool here = kk < key[s];
+        key[s] = sh ? key[s - 1] : (here ? kk : key[s]);
+        p[s] = sh ? p[s - 1] : (here ? pp : p[s]);
+      }
+      const bool h0 = kk < key[0];
+      key[0] = h0 ? kk : key[0];
+      p[0] = h0 ? pp : p[0];
+    } else {
+      // Sorted insert as a min/max network: slot s keeps the smaller of its
+      // key and max(slot s-1, kk) -- its own key below the insertion point,
+      // kk at it, the key of slot s-1 (shifted up) above it.
+      (void)pp;
+#pragma unroll
+      for (int s = K - 1; s > 0; --s) key[s] = kmin(key[s], kmax(key[s - 1], kk));
+      key[0] = kmin(key[0], kk);
     }
-    const bool h0 = kk < key[0];
-    key[0] = h0 ? kk : key[0];
-    if constexpr (POS) p[0] = h0 ? pp : p[0];
   }
 };
 

@@ -234,6 +234,28 @@ def test_knn_ties_duplicates_grid(gpu):
     assert np.array_equal(got, bw)
 
 
+def test_knn_float32_ties_and_denormal_distances(gpu):
+    """float32 trees order neighbours by a packed (distance bits, index) key:
+    exact ties on a duplicated integer grid, and a grid scaled by 3e-21 whose
+    squared distances (multiples of 9e-42) are float denormals."""
+    g = np.stack(np.meshgrid(np.arange(20), np.arange(20), np.arange(10)), -1).reshape(-1, 3)
+    grid = np.concatenate([g, g[::3], g[::7]]).astype(np.float32)
+    tiny = (g.astype(np.float32) * np.float32(3e-21)).astype(np.float32)
+    assert 0 < np.float32(3e-21) ** 2 < np.finfo(np.float32).tiny
+    kd = _kd()
+    for pts in (grid, tiny):
+        tree = kd._build(pts)
+        ot = O.OracleTree(pts)
+        for k in [1, 7, 16, 27, 40]:
+            got, cnt = _rows(kd._query(tree, None, k=k))
+            want, wcnt = ot.knn(None, k)
+            assert np.array_equal(cnt, wcnt), k
+            assert np.array_equal(got[:, :k], want), k
+        bw, _ = O.brute_knn(pts, pts[:400], 9, self_rows=np.arange(400))
+        got, _ = _rows(kd._query(tree, np.arange(400), k=9))
+        assert np.array_equal(got, bw)
+
+
 @pytest.mark.parametrize("dtype", [np.int8, np.int32, np.uint16, np.int64, np.uint64])
 def test_knn_integer_dtypes(gpu, dtype):
     rng = np.random.default_rng(8)

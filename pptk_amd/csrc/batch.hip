@@ -26,11 +26,13 @@
 // block to finish publishes the next level's range, so the host never reads
 // the node counter.
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 #include <mutex>
 #include <set>
 #include <vector>
 
+#include "coop.hpp"
 #include "levels.hpp"
 #include "morton.hpp"
 #include "pca.hpp"
@@ -382,22 +384,34 @@ __global__ __launch_bounds__(kBlock) void batch_query_kernel(TreeView<D, DIM> tv
                                                              D* __restrict__ out_eval,
                                                              unsigned long long* __restrict__ nfb) {
   const int64_t t = xcd_block() * blockDim.x + threadIdx.x;
-  if (t >= N) return;
-  const int64_t row = gorig[t];
-  const bool valid = t < (int64_t)*vtotal;
+  // Float forests search with the whole wave (coop.hpp): lanes past the end
+  // stay in the wave as inactive lanes.
+  constexpr bool kCoop = std::is_same<D, float>::value;
+  if (!kCoop && t >= N) return;
+  const bool valid = t < N && t < (int64_t)*vtotal;
+  const int64_t row = t < N ? gorig[t] : -1;
   // Positions are not carried through the top-k insertions: the normals
   // read each neighbour's tree position from the forward map (global row =
   // this point's global row - its cloud-local index + the neighbour's).
   TopK<D, K, false> st;
   st.init(kk);
   D q[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) q[d] = D(0);
   int32_t self = -1;
   if (valid) {
     const PointRec<D, DIM> r = tv.pts[t];
 #pragma unroll
     for (int d = 0; d < DIM; ++d) q[d] = r.c[d];
     self = r.orig;
-    knn_search_from_leaf<D, DIM>(tv, (int32_t)t, q, r2, (int32_t)t, kk, st);
+  }
+  if constexpr (kCoop) {
+    extern __shared__ uint64_t coop_lds[];
+    knn_coop_self<DIM, K>(tv, valid, (int32_t)t, q, r2, kk, true, st,
+                          coop_lds + (threadIdx.x >> 6) * (coop_wave_bytes<K, DIM>() / 8));
+    if (t >= N) return;
+  } else {
+    if (valid) knn_search_from_leaf<D, DIM>(tv, (int32_t)t, q, r2, (int32_t)t, kk, st);
   }
   if (out_idx) {
     int32_t c = 0;
@@ -540,7 +554,9 @@ void enqueue_run(pptk_batch* b, const D* pts, int32_t* out_idx, int32_t* out_cnt
       constexpr int K = decltype(kc)::value;
       constexpr int P = decltype(pc)::value;
       constexpr bool NM = decltype(nc)::value;
-      batch_query_kernel<D, DIM, K, P, NM><<<grid, b->qblock, 0, s>>>(
+      const size_t lds = std::is_same<D, float>::value
+                             ? (size_t)(b->qblock / 64) * coop_wave_bytes<K, DIM>() : 0;
+      batch_query_kernel<D, DIM, K, P, NM><<<grid, b->qblock, lds, s>>>(
           tv, b->N, vtotal, gorig, b->k, b->kk, b->nk, r2, out_idx, out_cnt, out_nrm, out_eval,
           nfb);
     };

@@ -19,7 +19,9 @@
 //           double-float covariance.
 //   FP64  : offsets, covariance and Jacobi in double.
 #include <algorithm>
+#include <type_traits>
 
+#include "coop.hpp"
 #include "pca.hpp"
 #include "query.hpp"
 #include "traverse.hpp"
@@ -101,6 +103,51 @@ __global__ __launch_bounds__(kBlock) void normals_knn_kernel(TreeView<D, 3> tv,
     float cf[3][3];
     knn_covariance<D, PREC>(tv, q, n, each, cd, cf);
     fb = solve_and_write<D, PREC, NE>(row, n, cd, cf, evecs, evals, sizes);
+  }
+  count_fallbacks(fb, nfb);
+}
+
+// Tree-order rows (rows == nullptr) of a float tree: the same as
+// normals_knn_kernel with the wave-cooperative search (coop.hpp). Lanes past
+// the end stay in the wave as inactive lanes: the search needs every lane.
+template <int K, int PREC, int NE>
+__global__ __launch_bounds__(kBlock) void normals_coop_kernel(TreeView<float, 3> tv, int64_t m,
+                                                              int k, float r2,
+                                                              float* __restrict__ evecs,
+                                                              float* __restrict__ evals,
+                                                              int32_t* __restrict__ sizes,
+                                                              unsigned long long* __restrict__ nfb) {
+  __shared__ uint64_t lds[kBlock / 64 * coop_wave_bytes<K, 3>() / 8];
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  float q[3] = {0.0f, 0.0f, 0.0f};
+  int64_t row = -1;
+  if (t < m && !(tv.n_dev && t >= *tv.n_dev)) {
+    const PointRec<float, 3> r = tv.pts[t];
+    row = r.orig;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) q[d] = r.c[d];
+  }
+  TopK<float, K, false> st;
+  st.init(k);
+  knn_coop_self<3, K>(tv, row >= 0, (int32_t)t, q, r2, k, false, st,
+                      lds + (threadIdx.x >> 6) * (coop_wave_bytes<K, 3>() / 8));
+  if (row < 0) return;
+  int n = 0;
+#pragma unroll
+  for (int s = 0; s < K; ++s) n += (s >= K - k && st.dist(s) < pos_inf<float>()) ? 1 : 0;
+  bool fb = false;
+  if (n == 0) {
+    write_nan<float, NE>(row, evecs, evals, sizes);
+  } else {
+    auto each = [&](auto f) {
+#pragma unroll
+      for (int s = 0; s < K; ++s)
+        if (s >= K - k && st.dist(s) < pos_inf<float>()) f(tv.fwd[st.idx(s)]);
+    };
+    double cd[3][3];
+    float cf[3][3];
+    knn_covariance<float, PREC>(tv, q, n, each, cd, cf);
+    fb = solve_and_write<float, PREC, NE>(row, n, cd, cf, evecs, evals, sizes);
   }
   count_fallbacks(fb, nfb);
 }
@@ -245,6 +292,14 @@ void normals_launch(const pptk_tree* t, int k, D r2, const int32_t* drows, int64
   } else if (k <= 32) {
     auto launch = [&](auto kc) {
       constexpr int K = decltype(kc)::value;
+      if constexpr (std::is_same<D, float>::value) {
+        if (!drows) {
+          normals_coop_kernel<K, PREC, NE><<<grid_of(m), kBlock, 0, s>>>(tv, m, k, r2, evecs,
+                                                                         evals, sizes, nfb);
+          PK_LAUNCH_CHECK();
+          return;
+        }
+      }
       normals_knn_kernel<D, K, PREC, NE><<<grid_of(m), kBlock, 0, s>>>(tv, drows, m, k, r2, evecs,
                                                                         evals, sizes, nfb);
       PK_LAUNCH_CHECK();

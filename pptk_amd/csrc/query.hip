@@ -8,8 +8,10 @@
 // Reference entry points: KdTree::KNearestNeighbors[Self] and
 // RNearNeighbors[Self] (pptk/kdtree/src/kdtree-impl.h:873-1069).
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
+#include "coop.hpp"
 #include "morton.hpp"
 #include "query.hpp"
 #include "sort.hpp"
@@ -53,6 +55,49 @@ __global__ __launch_bounds__(kBlock) void knn_kernel(TreeView<D, DIM> tv, int mo
       const bool f = ds < pos_inf<D>();
       orow[s - (K - k)] = f ? st.idx(s) : -1;
       if (out_dist) out_dist[row * k + (s - (K - k))] = f ? ds : pos_inf<D>();
+      c += f ? 1 : 0;
+    }
+  }
+  out_cnt[row] = c;
+}
+
+// kNN self queries over every point of a float tree (ALL mode): one
+// wave-cooperative search per wave (coop.hpp). Lanes past the end stay in
+// the wave as inactive lanes: the search needs every lane.
+template <int DIM, int K>
+__global__ __launch_bounds__(kBlock) void knn_coop_kernel(TreeView<float, DIM> tv, int64_t m,
+                                                          int64_t nrows, int k, float r2,
+                                                          int32_t* __restrict__ out_idx,
+                                                          int32_t* __restrict__ out_cnt,
+                                                          float* __restrict__ out_dist) {
+  __shared__ uint64_t lds[kBlock / 64 * coop_wave_bytes<K, DIM>() / 8];
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  float q[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) q[d] = 0.0f;
+  int64_t row = -1;
+  if (t < m && !(tv.n_dev && t >= *tv.n_dev)) {
+    const PointRec<float, DIM> r = tv.pts[t];
+    if (r.orig < nrows) {
+      row = r.orig;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) q[d] = r.c[d];
+    }
+  }
+  TopK<float, K, false> st;
+  st.init(k);
+  knn_coop_self<DIM, K>(tv, row >= 0, (int32_t)t, q, r2, k, true, st,
+                        lds + (threadIdx.x >> 6) * (coop_wave_bytes<K, DIM>() / 8));
+  if (row < 0) return;
+  int32_t c = 0;
+  int32_t* orow = out_idx + row * k;
+#pragma unroll
+  for (int s = 0; s < K; ++s) {
+    if (s >= K - k) {
+      const float ds = st.dist(s);
+      const bool f = ds < pos_inf<float>();
+      orow[s - (K - k)] = f ? st.idx(s) : -1;
+      if (out_dist) out_dist[row * k + (s - (K - k))] = f ? ds : pos_inf<float>();
       c += f ? 1 : 0;
     }
   }
@@ -300,21 +345,6 @@ __global__ __launch_bounds__(kBlock) void rnear_kernel(TreeView<D, DIM> tv, int 
 //     ranked with a ballot; at an inner node the wave descends.
 // Each lane's output is in tree order, exactly as rnear_kernel's.
 // Value of lane L (wave-uniform) in a scalar register: v_readlane, no LDS.
-__device__ __forceinline__ int32_t rl_i32(int32_t x, int L) {
-  return __builtin_amdgcn_readlane(x, L);
-}
-__device__ __forceinline__ int64_t rl_i64(int64_t x, int L) {
-  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int32_t)(uint32_t)x, L);
-  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int32_t)(uint32_t)((uint64_t)x >> 32), L);
-  return (int64_t)(((uint64_t)hi << 32) | lo);
-}
-__device__ __forceinline__ float rl(float x, int L) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), L));
-}
-__device__ __forceinline__ double rl(double x, int L) {
-  return __longlong_as_double(rl_i64(__double_as_longlong(x), L));
-}
-
 template <typename D, int DIM, bool FILL>
 __global__ __launch_bounds__(kBlock) void rnear_packet_kernel(TreeView<D, DIM> tv, int mode,
                                                               const int32_t* __restrict__ qlist,
@@ -604,6 +634,14 @@ void knn_run(const pptk_tree* t, const pptk_queries* qs, int k, double dmax, int
     if (k <= 32) {
       auto launch = [&](auto kc) {
         constexpr int K = decltype(kc)::value;
+        if constexpr (std::is_same<D, float>::value) {
+          if (pq.mode == kQueryAll) {
+            knn_coop_kernel<DIM, K><<<grid_of(m), kBlock, 0, s>>>(
+                tv, m, nrows, k, r2, (int32_t*)oi.dptr, (int32_t*)oc.dptr, (float*)od.dptr);
+            PK_LAUNCH_CHECK();
+            return;
+          }
+        }
         knn_kernel<D, DIM, K><<<grid_of(m), kBlock, 0, s>>>(
             tv, pq.mode, pq.dlist, qp, pq.order, m, nrows, k, r2, (int32_t*)oi.dptr, (int32_t*)oc.dptr,
             (D*)od.dptr);

@@ -68,6 +68,41 @@ struct TopK {
   }
 };
 
+// Unsigned 64-bit min / max of two packed float keys (below) in one
+// instruction. Every key read as an IEEE double is non-negative and finite:
+// its top 32 bits are a squared distance's float bits (sign 0, at most +inf =
+// 0x7f800000, so the double exponent stays below 0x7ff), and non-negative
+// doubles order as their bit patterns. Denormal doubles (distances below
+// 2^-126) are exact because f64 denormals are preserved. Written in asm
+// because the builtins first canonicalise each operand (one more
+// instruction each).
+__device__ __forceinline__ uint64_t key_min(uint64_t a, uint64_t b) {
+  uint64_t r;
+  asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+__device__ __forceinline__ uint64_t key_max(uint64_t a, uint64_t b) {
+  uint64_t r;
+  asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+
+// Lane L's value of x, as a wave-uniform value.
+__device__ __forceinline__ int32_t rl_i32(int32_t x, int L) {
+  return __builtin_amdgcn_readlane(x, L);
+}
+__device__ __forceinline__ int64_t rl_i64(int64_t x, int L) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int32_t)(uint32_t)x, L);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int32_t)(uint32_t)((uint64_t)x >> 32), L);
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ float rl(float x, int L) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), L));
+}
+__device__ __forceinline__ double rl(double x, int L) {
+  return __longlong_as_double(rl_i64(__double_as_longlong(x), L));
+}
+
 // Float specialisation: (d, o) packed into one 64-bit key, d's IEEE bits
 // above the original index. Squared distances are +0 or positive, so the
 // unsigned key order equals the (d, o) order and each slot update is one
@@ -98,24 +133,6 @@ struct TopK<float, K, POS> {
   }
   __device__ __forceinline__ bool beats(float dd, int32_t oo) const { return make(dd, oo) < key[K - 1]; }
 
-  // Unsigned 64-bit min / max of two keys in one instruction. Every key read
-  // as an IEEE double is non-negative and finite: its top 32 bits are a
-  // squared distance's float bits (sign 0, at most +inf = 0x7f800000, so the
-  // double exponent stays below 0x7ff), and non-negative doubles order as
-  // their bit patterns. Denormal doubles (distances below 2^-126) are exact
-  // because f64 denormals are preserved. Written in asm because the
-  // builtins first canonicalise each operand (one more instruction each).
-  static __device__ __forceinline__ uint64_t kmin(uint64_t a, uint64_t b) {
-    uint64_t r;
-    asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
-    return r;
-  }
-  static __device__ __forceinline__ uint64_t kmax(uint64_t a, uint64_t b) {
-    uint64_t r;
-    asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
-    return r;
-  }
-
   __device__ __forceinline__ void insert(float dd, int32_t oo, int32_t pp) {
     const uint64_t kk = make(dd, oo);
     if constexpr (POS) {
@@ -135,8 +152,8 @@ struct TopK<float, K, POS> {
       // kk at it, the key of slot s-1 (shifted up) above it.
       (void)pp;
 #pragma unroll
-      for (int s = K - 1; s > 0; --s) key[s] = kmin(key[s], kmax(key[s - 1], kk));
-      key[0] = kmin(key[0], kk);
+      for (int s = K - 1; s > 0; --s) key[s] = key_min(key[s], key_max(key[s - 1], kk));
+      key[0] = key_min(key[0], kk);
     }
   }
 };

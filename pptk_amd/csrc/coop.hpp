@@ -8,8 +8,8 @@
 //   * a node is entered when any lane's query can still improve from it
 //     (box lower bound <= that lane's bound, below);
 //   * a leaf's points are loaded 64 at a time, one per lane, into LDS, and
-//     filtered against the wave's query bounding box and the largest bound
-//     of its lanes; each surviving point is read back by every lane (one
+//     filtered against the bounding box of the queries whose ball reaches
+//     the leaf and the largest bound among them; each surviving point is read back by every lane (one
 //     broadcast LDS read), every lane computes its own distance, and a lane
 //     whose key is below its bound appends the key to its own buffer in LDS
 //     (branch-free: the slot past the last key always holds +inf);
@@ -293,8 +293,9 @@ __device__ __forceinline__ void knn_coop_self(const TreeView<float, DIM>& tv, bo
   // registers.
   uint64_t todo = __ballot(active);
   bool part = false;
-  BoxRec<float, DIM> qb{};
-  float rmax = -1.0f;
+  BoxRec<float, DIM> lqb{};       // bounding box of the queries that need the current leaf
+  bool inleaf = false;            // this lane needs the current leaf
+  float rmax = -1.0f;             // largest bound among them
   int32_t mid = 0, node = 0, depth = 0;
   uint64_t flo = 0, fhi = 0;
   int32_t c0 = 0, cend = 0, cbase = 0;   // leaf chunks still to load: [c0, cend)
@@ -324,7 +325,7 @@ __device__ __forceinline__ void knn_coop_self(const TreeView<float, DIM>& tv, bo
         PointRec<float, DIM> rec{};
         if (valid) rec = tv.pts[p];
         cs.chunk[cs.lane] = rec;
-        const float md = box_min_dist2<float, DIM>(qb, rec.c);
+        const float md = box_min_dist2<float, DIM>(lqb, rec.c);
         mask = __ballot(valid && md <= rmax);
         cbase = c0;
         c0 += 64;
@@ -341,12 +342,6 @@ __device__ __forceinline__ void knn_coop_self(const TreeView<float, DIM>& tv, bo
         const uint64_t pm = __ballot(part);
         todo &= ~pm;
         cs.refresh(part);
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) {
-          qb.lo[d] = uniform_f(wave_min_f(part ? q[d] : pos_inf<float>()));
-          qb.hi[d] = uniform_f(wave_max_f(part ? q[d] : -pos_inf<float>()));
-        }
-        rmax = uniform_f(wave_max_f(cs.bd));
         mid = (rl_i32(pos, __builtin_ctzll(pm)) + rl_i32(pos, 63 - __builtin_clzll(pm))) >> 1;
         node = R;
         depth = 0;
@@ -362,9 +357,21 @@ __device__ __forceinline__ void knn_coop_self(const TreeView<float, DIM>& tv, bo
       const NodeRec nr = tv.node[node];
       c0 = nr.a;
       cend = nr.b;
+      // the leaf's points are filtered against the lanes that need the leaf
+      // only: any other lane is farther from every point than its bound
+      {
+        const BoxRec<float, DIM> lb = tv.box[node];
+        inleaf = box_min_dist2<float, DIM>(lb, q) <= cs.bd;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+          lqb.lo[d] = uniform_f(wave_min_f(inleaf ? q[d] : pos_inf<float>()));
+          lqb.hi[d] = uniform_f(wave_max_f(inleaf ? q[d] : -pos_inf<float>()));
+        }
+        rmax = uniform_f(wave_max_f(inleaf ? cs.bd : -1.0f));
+      }
     }
     cs.flush(part);
-    rmax = uniform_f(wave_max_f(cs.bd));
+    rmax = uniform_f(wave_max_f(inleaf ? cs.bd : -1.0f));
   } while (!done);
 }
 

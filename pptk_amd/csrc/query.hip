@@ -344,8 +344,17 @@ __global__ __launch_bounds__(kBlock) void rnear_kernel(TreeView<D, DIM> tv, int 
 //     and tested against each straddling lane's query in turn, the hits
 //     ranked with a ballot; at an inner node the wave descends.
 // Each lane's output is in tree order, exactly as rnear_kernel's.
-// Value of lane L (wave-uniform) in a scalar register: v_readlane, no LDS.
-template <typename D, int DIM, bool FILL>
+// Number of set bits of mask in the lanes below this one (v_mbcnt).
+__device__ __forceinline__ int32_t rank_below(uint64_t mask) {
+  return (int32_t)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                            __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// EXCL: self queries (tree points): the query's own position is skipped.
+// Lane values needed by the whole wave are read with v_readlane (scalar
+// registers, no LDS); the fill keeps one 64-bit write pointer and end per
+// lane, and the hit tests are branch-free.
+template <typename D, int DIM, bool FILL, bool EXCL>
 __global__ __launch_bounds__(kBlock) void rnear_packet_kernel(TreeView<D, DIM> tv, int mode,
                                                               const int32_t* __restrict__ qlist,
                                                               const D* __restrict__ qpts,
@@ -356,7 +365,6 @@ __global__ __launch_bounds__(kBlock) void rnear_packet_kernel(TreeView<D, DIM> t
                                                               int32_t* __restrict__ out) {
   const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const int lane = threadIdx.x & 63;
-  const uint64_t lt_mask = (1ull << lane) - 1ull;
   D q[DIM];
 #pragma unroll
   for (int d = 0; d < DIM; ++d) q[d] = D(0);
@@ -366,31 +374,26 @@ __global__ __launch_bounds__(kBlock) void rnear_packet_kernel(TreeView<D, DIM> t
   if (t < m)
     has_row = resolve_query<D, DIM>(tv, mode, qlist, qpts, order, t, nrows, q, excl, row, ok);
   const bool active = has_row && ok;
-  int64_t base = 0, cap = 0;
+  int64_t wp = 0, wend = 0;         // fill: next output slot and end of this query's list
   if (FILL && active) {
-    base = offsets[row];
-    cap = offsets[row + 1] - base;
+    wp = offsets[row];
+    wend = offsets[row + 1];
   }
-  int32_t cnt = 0;                  // outputs so far for this lane's query
+  int32_t cnt = 0;                  // count: outputs so far for this lane's query
   int32_t done_end = INT32_MIN;     // nodes starting before this are taken
   int32_t node = __ballot(active) ? 0 : -1;
   while (node >= 0) {
     node = __builtin_amdgcn_readfirstlane(node);
     const NodeRec nr = tv.node[node];
     const BoxRec<D, DIM> bx = tv.box[node];
-    const bool hit = active && nr.a >= done_end && box_min_dist2<D, DIM>(bx, q) < r2;
+    const bool hit = active & (nr.a >= done_end) & (box_min_dist2<D, DIM>(bx, q) < r2);
     bool descend = false;
     if (__ballot(hit)) {
-      const bool inside = hit && box_max_dist2<D, DIM>(bx, q) < r2;
-      const bool strad = hit && !inside;
+      const bool inside = hit & (box_max_dist2<D, DIM>(bx, q) < r2);
+      const bool strad = hit & !inside;
       const uint64_t insm = __ballot(inside);
       const uint64_t strm = __ballot(strad);
-      const int32_t start = cnt;    // this lane's output position for the node
-      if (inside) {
-        const bool ex = excl >= nr.a && excl < nr.b;
-        cnt += (nr.b - nr.a) - (ex ? 1 : 0);
-        done_end = nr.b;
-      }
+      if (inside) done_end = nr.b;
       if constexpr (FILL) {
         if (insm) {
           // copy the range once per chunk, into every inside lane's list
@@ -402,17 +405,30 @@ __global__ __launch_bounds__(kBlock) void rnear_packet_kernel(TreeView<D, DIM> t
             while (lm) {
               const int L = __builtin_ctzll(lm);
               lm &= lm - 1ull;
-              const int32_t exL = rl_i32(excl, L);
-              const int32_t stL = rl_i32(start, L);
-              const int64_t bL = rl_i64(base, L);
-              const int64_t cL = rl_i64(cap, L);
-              if (valid && p != exL) {
-                const bool ex_in = exL >= nr.a && exL < nr.b;
-                const int64_t o = (int64_t)stL + (p - nr.a) - ((ex_in && p > exL) ? 1 : 0);
-                if (o < cL) out[bL + o] = og;
+              const int64_t wL = rl_i64(wp, L);
+              const int64_t eL = rl_i64(wend, L);
+              int64_t o = wL + (p - nr.a);
+              bool w = valid;
+              if constexpr (EXCL) {
+                const int32_t exL = rl_i32(excl, L);
+                // the own position is skipped: later points move down one slot
+                o -= (exL >= nr.a) & (p > exL) ? 1 : 0;
+                w &= p != exL;
               }
+              if (w & (o < eL)) out[o] = og;
             }
           }
+        }
+        if (inside) {
+          int32_t size = nr.b - nr.a;
+          if constexpr (EXCL) size -= (excl >= nr.a) & (excl < nr.b) ? 1 : 0;
+          wp += size;
+        }
+      } else {
+        if (inside) {
+          int32_t size = nr.b - nr.a;
+          if constexpr (EXCL) size -= (excl >= nr.a) & (excl < nr.b) ? 1 : 0;
+          cnt += size;
         }
       }
       if (strm) {
@@ -429,17 +445,16 @@ __global__ __launch_bounds__(kBlock) void rnear_packet_kernel(TreeView<D, DIM> t
               D qL[DIM];
 #pragma unroll
               for (int d = 0; d < DIM; ++d) qL[d] = rl(q[d], L);
-              const int32_t exL = rl_i32(excl, L);
-              const bool pass = valid && p != exL && dist2<D, DIM>(qL, rec.c) < r2;
+              bool pass = valid & (dist2<D, DIM>(qL, rec.c) < r2);
+              if constexpr (EXCL) pass &= p != rl_i32(excl, L);
               const uint64_t hits = __ballot(pass);
               if constexpr (FILL) {
-                const int32_t cntL = rl_i32(cnt, L);
-                const int64_t bL = rl_i64(base, L);
-                const int64_t cL = rl_i64(cap, L);
-                const int64_t o = (int64_t)cntL + __popcll(hits & lt_mask);
-                if (pass && o < cL) out[bL + o] = rec.orig;
+                const int64_t o = rl_i64(wp, L) + rank_below(hits);
+                if (pass & (o < rl_i64(wend, L))) out[o] = rec.orig;
+                if (lane == L) wp += __popcll(hits);
+              } else {
+                if (lane == L) cnt += __popcll(hits);
               }
-              if (lane == L) cnt += __popcll(hits);
             }
           }
         } else {
@@ -685,8 +700,12 @@ void rnear_count_run(const pptk_tree* t, const pptk_queries* qs, double dmax, in
   if (nrows > 0) PK_HIP(hipMemsetAsync(counts.get(), 0, (size_t)nrows * 4, s));
   const TreeView<D, DIM> tv = tree_view<D, DIM>(t);
   if (pq.m > 0) {
-    if (tv.nroots == 1)
-      rnear_packet_kernel<D, DIM, false><<<grid_of(pq.m), kBlock, 0, s>>>(
+    if (tv.nroots == 1 && pq.mode == kQueryPoints)
+      rnear_packet_kernel<D, DIM, false, false><<<grid_of(pq.m), kBlock, 0, s>>>(
+          tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.order, pq.m, nrows, r2,
+          counts.as<int32_t>(), nullptr, nullptr);
+    else if (tv.nroots == 1)
+      rnear_packet_kernel<D, DIM, false, true><<<grid_of(pq.m), kBlock, 0, s>>>(
           tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.order, pq.m, nrows, r2,
           counts.as<int32_t>(), nullptr, nullptr);
     else
@@ -730,8 +749,12 @@ void rnear_fill_run(const pptk_tree* t, const pptk_queries* qs, double dmax, con
   if (total > 0 && !out_idx) fail(PPTK_E_VALUE, "null output");
   const TreeView<D, DIM> tv = tree_view<D, DIM>(t);
   if (pq.m > 0 && total > 0) {
-    if (tv.nroots == 1)
-      rnear_packet_kernel<D, DIM, true><<<grid_of(pq.m), kBlock, 0, s>>>(
+    if (tv.nroots == 1 && pq.mode == kQueryPoints)
+      rnear_packet_kernel<D, DIM, true, false><<<grid_of(pq.m), kBlock, 0, s>>>(
+          tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.order, pq.m, nrows, r2, nullptr,
+          (const int64_t*)doff.ptr, (int32_t*)oi.dptr);
+    else if (tv.nroots == 1)
+      rnear_packet_kernel<D, DIM, true, true><<<grid_of(pq.m), kBlock, 0, s>>>(
           tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.order, pq.m, nrows, r2, nullptr,
           (const int64_t*)doff.ptr, (int32_t*)oi.dptr);
     else

@@ -64,10 +64,10 @@ def algorithmic_bytes(n, k, m_rnear, rnear_total):
 
 # The kernel that dominates each phase (names as rocprofv3 reports them).
 PHASE_KERNEL = {
-    "knn": "knn_kernel<float, 3, 16>",
-    "rnear_count": "rnear_kernel<float, 3, false>",
-    "rnear_fill": "rnear_kernel<float, 3, true>",
-    "normals": "normals_knn_kernel<float, 16, 1, 1>",
+    "knn": "knn_coop_kernel<3, 16>",
+    "rnear_count": "rnear_packet_kernel<float, 3, false, false>",
+    "rnear_fill": "rnear_packet_kernel<float, 3, true, false>",
+    "normals": "normals_coop_kernel<16, 1, 1>",
 }
 
 
@@ -457,7 +457,7 @@ def bench_served(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", default="lidar", choices=["lidar", "batched", "served"])
-    ap.add_argument("--rates", default="400,800,1000,1200,1400,1600,2000",
+    ap.add_argument("--rates", default="400,800,1200,1600,2000,2400,2800,3200,4000",
                     help="served: offered request rates (requests/s)")
     ap.add_argument("--requests", type=int, default=200, help="served: requests per rate")
     ap.add_argument("--pool", type=int, default=40, help="served: distinct clouds")

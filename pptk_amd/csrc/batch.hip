@@ -407,7 +407,7 @@ __global__ __launch_bounds__(kBlock) void batch_query_kernel(TreeView<D, DIM> tv
   }
   if constexpr (kCoop) {
     extern __shared__ uint64_t coop_lds[];
-    knn_coop_self<DIM, K>(tv, valid, (int32_t)t, q, r2, kk, true, st,
+    knn_coop_self<DIM, K>(tv, valid, (int32_t)t, self, q, r2, kk, true, st,
                           coop_lds + (threadIdx.x >> 6) * (coop_wave_bytes<K, DIM>() / 8));
     if (t >= N) return;
   } else {

@@ -110,6 +110,11 @@ __device__ __forceinline__ float wave_min_f(float x) {
   return a < b ? a : b;
 }
 
+// Clears bit j of a wave-uniform 64-bit mask (one scalar instruction).
+__device__ __forceinline__ void clear_bit(uint64_t& mask, int j) {
+  asm("s_bitset0_b64 %0, %1" : "+s"(mask) : "s"(j));
+}
+
 // x as a wave-uniform (scalar) value; x must be equal in every active lane.
 __device__ __forceinline__ float uniform_f(float x) {
   return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
@@ -226,16 +231,16 @@ struct CoopSearch {
   }
 };
 
-// k-nearest search for the tree point at position pos, done by the whole
-// wave. Every lane of the wave must call it (inactive lanes with
-// active = false). excl_self: the point itself is not a neighbour (kNN self
-// queries); otherwise it is one (normals). st must be freshly init(k);
+// k-nearest search for the tree point at position pos (record index
+// self_orig), done by the whole wave. Every lane of the wave must call it
+// (inactive lanes with active = false). excl_self: the point itself is not a
+// neighbour (kNN self queries); otherwise it is one (normals). st must be freshly init(k);
 // wave_lds is this wave's coop_wave_bytes<K, DIM>() of LDS.
 template <int DIM, int K>
 __device__ __forceinline__ void knn_coop_self(const TreeView<float, DIM>& tv, bool active,
-                                              int32_t pos, const float* q, float r2, int k,
-                                              bool excl_self, TopK<float, K, false>& st,
-                                              uint64_t* wave_lds) {
+                                              int32_t pos, int32_t self_orig, const float* q,
+                                              float r2, int k, bool excl_self,
+                                              TopK<float, K, false>& st, uint64_t* wave_lds) {
   CoopSearch<DIM, K> cs(tv, st, wave_lds, q);
   constexpr int M = CoopSearch<DIM, K>::M;
 #pragma unroll
@@ -304,21 +309,21 @@ __device__ __forceinline__ void knn_coop_self(const TreeView<float, DIM>& tv, bo
   do {
     for (;;) {
       bool full = false;
-      while (mask) {
-        const int j = __builtin_ctzll(mask);
-        mask &= mask - 1ull;
-        const PointRec<float, DIM> cr = cs.chunk[j];
-        const float dd = dist2<float, DIM>(q, cr.c);
-        const uint64_t key = ((uint64_t)__float_as_uint(dd) << 32) | (uint32_t)cr.orig;
-        const bool take = key < cs.bkey && !(excl_self && cbase + j == pos);
-        cs.lbuf[cs.cnt * 64 + cs.lane] = take ? key : kKeyEmpty;
-        cs.cnt += take ? 1 : 0;
-        if (__ballot(cs.cnt == M)) {
-          full = true;
-          break;
-        }
+      if (mask) {
+        do {
+          const int j = __builtin_ctzll(mask);
+          clear_bit(mask, j);
+          const PointRec<float, DIM> cr = cs.chunk[j];
+          const float dd = dist2<float, DIM>(q, cr.c);
+          const uint64_t key = ((uint64_t)__float_as_uint(dd) << 32) | (uint32_t)cr.orig;
+          // the own point has the lane's own index (unique in its cloud)
+          const bool take = (key < cs.bkey) & !(excl_self & (cr.orig == self_orig));
+          cs.lbuf[cs.cnt * 64 + cs.lane] = take ? key : kKeyEmpty;
+          cs.cnt += take ? 1 : 0;
+          full = __ballot(cs.cnt == M) != 0ull;
+        } while ((mask != 0ull) & !full);
+        if (full) break;
       }
-      if (full) break;
       if (c0 < cend) {
         const int32_t p = c0 + cs.lane;
         const bool valid = p < cend;

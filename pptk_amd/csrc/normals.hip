@@ -129,7 +129,7 @@ __global__ __launch_bounds__(kBlock) void normals_coop_kernel(TreeView<float, 3>
   }
   TopK<float, K, false> st;
   st.init(k);
-  knn_coop_self<3, K>(tv, row >= 0, (int32_t)t, q, r2, k, false, st,
+  knn_coop_self<3, K>(tv, row >= 0, (int32_t)t, (int32_t)row, q, r2, k, false, st,
                       lds + (threadIdx.x >> 6) * (coop_wave_bytes<K, 3>() / 8));
   if (row < 0) return;
   int n = 0;

@@ -86,7 +86,7 @@ __global__ __launch_bounds__(kBlock) void knn_coop_kernel(TreeView<float, DIM> t
   }
   TopK<float, K, false> st;
   st.init(k);
-  knn_coop_self<DIM, K>(tv, row >= 0, (int32_t)t, q, r2, k, true, st,
+  knn_coop_self<DIM, K>(tv, row >= 0, (int32_t)t, (int32_t)row, q, r2, k, true, st,
                         lds + (threadIdx.x >> 6) * (coop_wave_bytes<K, DIM>() / 8));
   if (row < 0) return;
   int32_t c = 0;

@@ -11,7 +11,7 @@
 //
 // Forest layout (same records as a single tree, see build.hip):
 //   * sort key = cloud id above the cloud's Morton code (mb bits per axis on
-//     the cloud's own cubic frame); rows with NaN/inf get cloud id C and sort
+//     the cloud's own cubic frame, mb from the largest cloud's size); rows with NaN/inf get cloud id C and sort
 //     after every valid row, so no compaction pass is needed;
 //   * nodes [0, C) are the cloud roots (parent -1); when C is even, node C is
 //     an empty placeholder root so that child pairs start at an odd id;
@@ -660,7 +660,14 @@ pptk_status api_batch_create(const int64_t* sizes, int64_t C, int dim, pptk_dtyp
     b->leaf = std::min(256, std::max(1, leaf));
     const int cbits = bits_for(C);  // room for ids 0..C (C = invalid rows)
     const int mbmax = dim == 2 ? 31 : dim == 3 ? 21 : 15;
-    b->mb = std::min(mbmax, (63 - cbits) / dim);
+    // Morton bits per axis: enough to give the largest cloud ~32x more cells
+    // per axis than its points need on a surface (ceil(log2 n / dim) + 5), so
+    // equal codes stay rare; fewer bits mean fewer radix passes and levels.
+    // Results do not depend on it (the searches are exact for any tree).
+    int64_t maxn = 1;
+    for (int64_t c = 0; c < C; ++c) maxn = std::max<int64_t>(maxn, off[c + 1] - off[c]);
+    const int need = (bits_for(maxn) + dim - 1) / dim + 5;
+    b->mb = std::min({mbmax, (63 - cbits) / dim, need});
     b->shift = dim * b->mb;
     b->end_bit = b->shift + cbits;
     b->nlev = b->shift + 33;

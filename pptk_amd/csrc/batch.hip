@@ -405,10 +405,11 @@ __global__ __launch_bounds__(kBlock) void batch_query_kernel(TreeView<D, DIM> tv
     for (int d = 0; d < DIM; ++d) q[d] = r.c[d];
     self = r.orig;
   }
+  uint64_t* wl = nullptr;
   if constexpr (kCoop) {
     extern __shared__ uint64_t coop_lds[];
-    knn_coop_self<DIM, K>(tv, valid, (int32_t)t, self, q, r2, kk, true, st,
-                          coop_lds + (threadIdx.x >> 6) * (coop_wave_bytes<K, DIM>() / 8));
+    wl = coop_lds + (threadIdx.x >> 6) * (coop_wave_bytes<K, DIM>() / 8);
+    knn_coop_self<DIM, K>(tv, valid, (int32_t)t, self, q, r2, kk, true, st, wl);
     if (t >= N) return;
   } else {
     if (valid) knn_search_from_leaf<D, DIM>(tv, (int32_t)t, q, r2, (int32_t)t, kk, st);
@@ -437,7 +438,8 @@ __global__ __launch_bounds__(kBlock) void batch_query_kernel(TreeView<D, DIM> tv
     if (n == 0) {
       write_nan<D, 1>(row, out_nrm, out_eval, nullptr);
     } else {
-      auto each = [&](auto f) {
+      // neighbourhood positions in order: the point itself at its rank
+      auto merged = [&](auto f) {
         bool pending = self_in;
         int taken = 0;
 #pragma unroll
@@ -456,7 +458,22 @@ __global__ __launch_bounds__(kBlock) void batch_query_kernel(TreeView<D, DIM> tv
       };
       double cd[3][3];
       float cf[3][3];
-      knn_covariance<D, PREC>(tv, q, n, each, cd, cf);
+      if constexpr (kCoop) {
+        // staged in this lane's LDS buffer (free after the search), so the
+        // keys are dead before the covariance's two passes
+        int32_t* pl = coop_positions(wl);
+        int m = 0;
+        merged([&](int32_t p) {
+          pl[m * 64] = p;
+          ++m;
+        });
+        auto each = [&](auto f) {
+          for (int j = 0; j < m; ++j) f(pl[j * 64]);
+        };
+        knn_covariance<D, PREC>(tv, q, n, each, cd, cf);
+      } else {
+        knn_covariance<D, PREC>(tv, q, n, merged, cd, cf);
+      }
       fb = solve_and_write<D, PREC, 1>(row, n, cd, cf, out_nrm, out_eval, nullptr);
     }
     count_fallbacks(fb, nfb);

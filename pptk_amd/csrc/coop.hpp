@@ -35,9 +35,21 @@ constexpr uint64_t kKeyEmpty = (0x7f800000ull << 32) | 0xffffffffull;
 
 // Buffer slots per lane for a top-k of K: 8 (4 for K = 4).
 template <int K> constexpr int coop_slots() { return K < 8 ? K : 8; }
-// LDS bytes for one wave: the lanes' buffers, then one leaf chunk of records.
+// LDS bytes of one wave's buffer area: the lanes' key buffers during the
+// search; afterwards the same bytes hold up to K + 1 neighbour positions per
+// lane (coop_positions).
+template <int K> constexpr int coop_buf_bytes() {
+  return coop_slots<K>() * 64 * 8 > ((K + 1) * 64 * 4 + 15) / 16 * 16
+             ? coop_slots<K>() * 64 * 8
+             : ((K + 1) * 64 * 4 + 15) / 16 * 16;
+}
+// LDS bytes for one wave: the buffer area, then one leaf chunk of records.
 template <int K, int DIM> constexpr int coop_wave_bytes() {
-  return coop_slots<K>() * 64 * 8 + 64 * (int)sizeof(PointRec<float, DIM>);
+  return coop_buf_bytes<K>() + 64 * (int)sizeof(PointRec<float, DIM>);
+}
+// After the search: slot j of this lane's neighbour positions (j <= K).
+__device__ __forceinline__ int32_t* coop_positions(uint64_t* wave_lds) {
+  return (int32_t*)wave_lds + (threadIdx.x & 63);
 }
 
 // Compare-exchange of two keys (a <= b after). One asm block, b updated in
@@ -154,7 +166,8 @@ struct CoopSearch {
 
   __device__ __forceinline__ CoopSearch(const TreeView<float, DIM>& tv_, TopK<float, K, false>& st_,
                                         uint64_t* wave_lds, const float* q_)
-      : tv(tv_), st(st_), lbuf(wave_lds), chunk((PointRec<float, DIM>*)(wave_lds + M * 64)),
+      : tv(tv_), st(st_), lbuf(wave_lds),
+        chunk((PointRec<float, DIM>*)((char*)wave_lds + coop_buf_bytes<K>())),
         q(q_), lane(threadIdx.x & 63), cnt(0), capkey(0), bkey(0), bd(-1.0f) {}
 
   __device__ __forceinline__ void refresh(bool part) {

@@ -129,20 +129,26 @@ __global__ __launch_bounds__(kBlock) void normals_coop_kernel(TreeView<float, 3>
   }
   TopK<float, K, false> st;
   st.init(k);
-  knn_coop_self<3, K>(tv, row >= 0, (int32_t)t, (int32_t)row, q, r2, k, false, st,
-                      lds + (threadIdx.x >> 6) * (coop_wave_bytes<K, 3>() / 8));
+  uint64_t* wl = lds + (threadIdx.x >> 6) * (coop_wave_bytes<K, 3>() / 8);
+  knn_coop_self<3, K>(tv, row >= 0, (int32_t)t, (int32_t)row, q, r2, k, false, st, wl);
   if (row < 0) return;
+  // The neighbours' tree positions, in key order, go to this lane's LDS
+  // buffer: the keys are dead before the covariance's two passes.
+  int32_t* pos = coop_positions(wl);
   int n = 0;
 #pragma unroll
-  for (int s = 0; s < K; ++s) n += (s >= K - k && st.dist(s) < pos_inf<float>()) ? 1 : 0;
+  for (int s = 0; s < K; ++s) {
+    if (s >= K - k && st.dist(s) < pos_inf<float>()) {
+      pos[n * 64] = tv.fwd[st.idx(s)];
+      ++n;
+    }
+  }
   bool fb = false;
   if (n == 0) {
     write_nan<float, NE>(row, evecs, evals, sizes);
   } else {
     auto each = [&](auto f) {
-#pragma unroll
-      for (int s = 0; s < K; ++s)
-        if (s >= K - k && st.dist(s) < pos_inf<float>()) f(tv.fwd[st.idx(s)]);
+      for (int j = 0; j < n; ++j) f(pos[j * 64]);
     };
     double cd[3][3];
     float cf[3][3];

@@ -122,9 +122,10 @@ __device__ __forceinline__ float wave_min_f(float x) {
   return a < b ? a : b;
 }
 
-// Clears bit j of a wave-uniform 64-bit mask (one scalar instruction).
-__device__ __forceinline__ void clear_bit(uint64_t& mask, int j) {
-  asm("s_bitset0_b64 %0, %1" : "+s"(mask) : "s"(j));
+// Number of set bits of mask in the lanes below this one (v_mbcnt).
+__device__ __forceinline__ int32_t rank_below(uint64_t mask) {
+  return (int32_t)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                            __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
 // x as a wave-uniform (scalar) value; x must be equal in every active lane.
@@ -316,17 +317,16 @@ __device__ __forceinline__ void knn_coop_self(const TreeView<float, DIM>& tv, bo
   float rmax = -1.0f;             // largest bound among them
   int32_t mid = 0, node = 0, depth = 0;
   uint64_t flo = 0, fhi = 0;
-  int32_t c0 = 0, cend = 0, cbase = 0;   // leaf chunks still to load: [c0, cend)
-  uint64_t mask = 0;                      // candidates of the loaded chunk
+  int32_t c0 = 0, cend = 0;       // leaf chunks still to load: [c0, cend)
+  int ci = 0, ncand = 0;          // next and number of candidates in the LDS chunk
   bool walking = false, done = false;
   do {
     for (;;) {
       bool full = false;
-      if (mask) {
+      if (ci < ncand) {
         do {
-          const int j = __builtin_ctzll(mask);
-          clear_bit(mask, j);
-          const PointRec<float, DIM> cr = cs.chunk[j];
+          const PointRec<float, DIM> cr = cs.chunk[ci];
+          ++ci;
           const float dd = dist2<float, DIM>(q, cr.c);
           const uint64_t key = ((uint64_t)__float_as_uint(dd) << 32) | (uint32_t)cr.orig;
           // the own point has the lane's own index (unique in its cloud)
@@ -334,18 +334,21 @@ __device__ __forceinline__ void knn_coop_self(const TreeView<float, DIM>& tv, bo
           cs.lbuf[cs.cnt * 64 + cs.lane] = take ? key : kKeyEmpty;
           cs.cnt += take ? 1 : 0;
           full = __ballot(cs.cnt == M) != 0ull;
-        } while ((mask != 0ull) & !full);
+        } while ((ci < ncand) & !full);
         if (full) break;
       }
       if (c0 < cend) {
+        // the chunk's surviving points are compacted, in tree order, to the
+        // front of the LDS chunk
         const int32_t p = c0 + cs.lane;
         const bool valid = p < cend;
         PointRec<float, DIM> rec{};
         if (valid) rec = tv.pts[p];
-        cs.chunk[cs.lane] = rec;
-        const float md = box_min_dist2<float, DIM>(lqb, rec.c);
-        mask = __ballot(valid && md <= rmax);
-        cbase = c0;
+        const bool pass = valid && box_min_dist2<float, DIM>(lqb, rec.c) <= rmax;
+        const uint64_t pm = __ballot(pass);
+        if (pass) cs.chunk[rank_below(pm)] = rec;
+        ncand = __popcll(pm);
+        ci = 0;
         c0 += 64;
         continue;
       }

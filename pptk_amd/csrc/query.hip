@@ -344,12 +344,6 @@ __global__ __launch_bounds__(kBlock) void rnear_kernel(TreeView<D, DIM> tv, int 
 //     and tested against each straddling lane's query in turn, the hits
 //     ranked with a ballot; at an inner node the wave descends.
 // Each lane's output is in tree order, exactly as rnear_kernel's.
-// Number of set bits of mask in the lanes below this one (v_mbcnt).
-__device__ __forceinline__ int32_t rank_below(uint64_t mask) {
-  return (int32_t)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
-                                            __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-}
-
 // EXCL: self queries (tree points): the query's own position is skipped.
 // Lane values needed by the whole wave are read with v_readlane (scalar
 // registers, no LDS); the fill keeps one 64-bit write pointer and end per

@@ -23,8 +23,10 @@ struct LevelState {
 };
 
 // Split every node of level l (the same rule as the single-tree build: the
-// highest differing key bit, or the middle when all keys are equal). The last
-// block to finish publishes level l+1 = the pairs this launch appended.
+// highest differing key bit, or the middle when all keys are equal). Only the
+// blocks the level's width needs take part (the rest return at once, so a
+// narrow level costs no more than a few blocks); the last of them to finish
+// publishes level l+1 = the pairs this launch appended.
 static __global__ __launch_bounds__(kLevelBlock) void split_level_dev_kernel(const uint64_t* __restrict__ keys,
                                                                  NodeRec* __restrict__ nodes,
                                                                  int32_t* __restrict__ lev_s,
@@ -34,7 +36,11 @@ static __global__ __launch_bounds__(kLevelBlock) void split_level_dev_kernel(con
                                                                  int64_t cap,
                                                                  LevelState* __restrict__ st) {
   const int32_t s = lev_s[l], e = lev_e[l];
-  const int64_t stride = (int64_t)gridDim.x * kLevelBlock;
+  const int64_t width = (int64_t)e - s;
+  const int64_t want = width > 0 ? (width + kLevelBlock - 1) / kLevelBlock : 1;
+  const unsigned used = (unsigned)(want < (int64_t)gridDim.x ? want : (int64_t)gridDim.x);
+  if (blockIdx.x >= used) return;
+  const int64_t stride = (int64_t)used * kLevelBlock;
   for (int64_t i = s + (int64_t)blockIdx.x * kLevelBlock + threadIdx.x; i < e; i += stride) {
     const int32_t a = nodes[i].a, b = nodes[i].b;
     if (b - a <= leaf) continue;  // child stays -1
@@ -68,7 +74,7 @@ static __global__ __launch_bounds__(kLevelBlock) void split_level_dev_kernel(con
   __syncthreads();
   if (threadIdx.x == 0) {
     __threadfence();
-    last = atomicAdd(&done[l], 1u) == gridDim.x - 1u;
+    last = atomicAdd(&done[l], 1u) == used - 1u;
   }
   __syncthreads();
   if (last && threadIdx.x == 0) {

@@ -368,10 +368,24 @@ __global__ __launch_bounds__(kBlock) void rnear_packet_kernel(TreeView<D, DIM> t
   if (t < m)
     has_row = resolve_query<D, DIM>(tv, mode, qlist, qpts, order, t, nrows, q, excl, row, ok);
   const bool active = has_row && ok;
-  int64_t wp = 0, wend = 0;         // fill: next output slot and end of this query's list
-  if (FILL && active) {
-    wp = offsets[row];
-    wend = offsets[row + 1];
+  // Lane values read by the whole wave live in LDS: the query (and its own
+  // position) and, for the fill, the next output slot and the list end; one
+  // broadcast read fetches a lane's record. The write pointers are kept only
+  // there.
+  __shared__ PointRec<D, DIM> qsh[kBlock];
+  __shared__ int64_t wsh[FILL ? 2 * kBlock : 2];
+  PointRec<D, DIM>* wq = qsh + (threadIdx.x & ~63);
+  int64_t* ww = wsh + (FILL ? 2 * (threadIdx.x & ~63) : 0);
+  {
+    PointRec<D, DIM> qr;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) qr.c[d] = q[d];
+    qr.orig = excl;
+    wq[lane] = qr;
+    if constexpr (FILL) {
+      ww[2 * lane] = active ? offsets[row] : 0;
+      ww[2 * lane + 1] = active ? offsets[row + 1] : 0;
+    }
   }
   int32_t cnt = 0;                  // count: outputs so far for this lane's query
   int32_t done_end = INT32_MIN;     // nodes starting before this are taken
@@ -399,12 +413,12 @@ __global__ __launch_bounds__(kBlock) void rnear_packet_kernel(TreeView<D, DIM> t
             while (lm) {
               const int L = __builtin_ctzll(lm);
               lm &= lm - 1ull;
-              const int64_t wL = rl_i64(wp, L);
-              const int64_t eL = rl_i64(wend, L);
+              const int64_t wL = ww[2 * L];
+              const int64_t eL = ww[2 * L + 1];
               int64_t o = wL + (p - nr.a);
               bool w = valid;
               if constexpr (EXCL) {
-                const int32_t exL = rl_i32(excl, L);
+                const int32_t exL = wq[L].orig;
                 // the own position is skipped: later points move down one slot
                 o -= (exL >= nr.a) & (p > exL) ? 1 : 0;
                 w &= p != exL;
@@ -416,7 +430,7 @@ __global__ __launch_bounds__(kBlock) void rnear_packet_kernel(TreeView<D, DIM> t
         if (inside) {
           int32_t size = nr.b - nr.a;
           if constexpr (EXCL) size -= (excl >= nr.a) & (excl < nr.b) ? 1 : 0;
-          wp += size;
+          ww[2 * lane] += size;
         }
       } else {
         if (inside) {
@@ -436,16 +450,15 @@ __global__ __launch_bounds__(kBlock) void rnear_packet_kernel(TreeView<D, DIM> t
             while (sm) {
               const int L = __builtin_ctzll(sm);
               sm &= sm - 1ull;
-              D qL[DIM];
-#pragma unroll
-              for (int d = 0; d < DIM; ++d) qL[d] = rl(q[d], L);
-              bool pass = valid & (dist2<D, DIM>(qL, rec.c) < r2);
-              if constexpr (EXCL) pass &= p != rl_i32(excl, L);
+              const PointRec<D, DIM> qL = wq[L];
+              bool pass = valid & (dist2<D, DIM>(qL.c, rec.c) < r2);
+              if constexpr (EXCL) pass &= p != qL.orig;
               const uint64_t hits = __ballot(pass);
               if constexpr (FILL) {
-                const int64_t o = rl_i64(wp, L) + rank_below(hits);
-                if (pass & (o < rl_i64(wend, L))) out[o] = rec.orig;
-                if (lane == L) wp += __popcll(hits);
+                const int64_t wL = ww[2 * L];
+                const int64_t o = wL + rank_below(hits);
+                if (pass & (o < ww[2 * L + 1])) out[o] = rec.orig;
+                if (lane == 0) ww[2 * L] = wL + __popcll(hits);
               } else {
                 if (lane == L) cnt += __popcll(hits);
               }

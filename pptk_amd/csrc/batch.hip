@@ -537,10 +537,14 @@ void enqueue_run(pptk_batch* b, const D* pts, int32_t* out_idx, int32_t* out_cnt
         b->lev_e.as<int32_t>(), b->done.as<uint32_t>(), l, b->nlev, b->leaf, b->cap, st);
     lc();
   }
+  leaf_box_kernel<D, DIM><<<4 * kLevelGrid, kBlock, 0, s>>>(
+      b->rec.as<PointRec<D, DIM>>(), b->nodes.as<NodeRec>(), b->box.as<BoxRec<D, DIM>>(),
+      b->leafof.as<int32_t>(), 0, &st->counter);
+  lc();
   for (int l = b->nlev - 1; l >= 0; --l) {
     box_level_dev_kernel<D, DIM><<<kLevelGrid, kBlock, 0, s>>>(
-        b->rec.as<PointRec<D, DIM>>(), b->nodes.as<NodeRec>(), b->box.as<BoxRec<D, DIM>>(),
-        b->leafof.as<int32_t>(), b->lev_s.as<int32_t>(), b->lev_e.as<int32_t>(), l);
+        b->nodes.as<NodeRec>(), b->box.as<BoxRec<D, DIM>>(), b->lev_s.as<int32_t>(),
+        b->lev_e.as<int32_t>(), l);
     lc();
   }
   if (b->timing) PK_HIP(hipEventRecord(b->ev[1], s));

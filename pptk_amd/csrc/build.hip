@@ -416,12 +416,19 @@ void build_impl(pptk_tree* t, const T* dpts, int64_t n, hipStream_t s) {
                         hipMemcpyDeviceToDevice, s));
   t->box.reset((size_t)nn * sizeof(BoxRec<D, DIM>));
   t->leafof.reset((size_t)std::max<int64_t>(nv, 1) * 4);
+  if (nv > 0) {
+    const unsigned g =
+        (unsigned)std::min<int64_t>(4096, std::max<int64_t>(1, ceil_div(nn * 8, kLevelBlock)));
+    leaf_box_kernel<D, DIM><<<g, kLevelBlock, 0, s>>>(
+        t->pts.as<PointRec<D, DIM>>(), t->node.as<NodeRec>(), t->box.as<BoxRec<D, DIM>>(),
+        t->leafof.as<int32_t>(), nn, nullptr);
+    PK_LAUNCH_CHECK();
+  }
   for (int l = nlev - 1; l >= 0; --l) {
     const int64_t width = he[l] - hs[l];
     const unsigned g = (unsigned)std::min<int64_t>(1024, std::max<int64_t>(1, ceil_div(width, kBlock)));
     box_level_dev_kernel<D, DIM><<<g, kLevelBlock, 0, s>>>(
-        t->pts.as<PointRec<D, DIM>>(), t->node.as<NodeRec>(), t->box.as<BoxRec<D, DIM>>(),
-        t->leafof.as<int32_t>(), lev_s, lev_e, l);
+        t->node.as<NodeRec>(), t->box.as<BoxRec<D, DIM>>(), lev_s, lev_e, l);
     PK_LAUNCH_CHECK();
   }
   PK_HIP(hipStreamSynchronize(s));  // temporaries return to the pool

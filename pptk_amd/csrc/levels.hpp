@@ -6,8 +6,9 @@
 // (at the highest key bit where its first and last keys differ, found by
 // binary search, or at the middle when all its keys are equal), appends the
 // children in adjacent pairs at the shared counter, and its last block to
-// finish publishes level l + 1. The box launches then run bottom-up over the
-// same level table. No host read is needed between levels.
+// finish publishes level l + 1. One launch then computes every leaf's box,
+// and the box launches combine child boxes bottom-up over the same level
+// table. No host read is needed between levels.
 #pragma once
 #include "common.hpp"
 
@@ -85,11 +86,64 @@ static __global__ __launch_bounds__(kLevelBlock) void split_level_dev_kernel(con
   }
 }
 
+// Boxes and leafof of every leaf, eight lanes per node (nodes [0, nn), nn
+// read from *nn_dev when given): the lanes of a leaf read its points eight
+// consecutive records at a time and combine their boxes with shuffles.
 template <typename D, int DIM>
-static __global__ __launch_bounds__(kLevelBlock) void box_level_dev_kernel(const PointRec<D, DIM>* __restrict__ rec,
-                                                               const NodeRec* __restrict__ nodes,
+static __global__ __launch_bounds__(kLevelBlock) void leaf_box_kernel(const PointRec<D, DIM>* __restrict__ rec,
+                                                          const NodeRec* __restrict__ nodes,
+                                                          BoxRec<D, DIM>* __restrict__ box,
+                                                          int32_t* __restrict__ leafof,
+                                                          int64_t nn,
+                                                          const int32_t* __restrict__ nn_dev) {
+  if (nn_dev) nn = *nn_dev;
+  const int sub = threadIdx.x & 7;
+  const int64_t groups = (int64_t)gridDim.x * (kLevelBlock / 8);
+  for (int64_t i = ((int64_t)blockIdx.x * kLevelBlock + threadIdx.x) >> 3; i < nn; i += groups) {
+    const NodeRec nr = nodes[i];
+    if (nr.child >= 0) continue;  // the same for the group's eight lanes
+    D lo[DIM], hi[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      lo[d] = pos_inf<D>();
+      hi[d] = -pos_inf<D>();
+    }
+    for (int32_t p = nr.a + sub; p < nr.b; p += 8) {
+      const PointRec<D, DIM> r = rec[p];
+      leafof[p] = (int32_t)i;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        lo[d] = r.c[d] < lo[d] ? r.c[d] : lo[d];
+        hi[d] = r.c[d] > hi[d] ? r.c[d] : hi[d];
+      }
+    }
+#pragma unroll
+    for (int o = 1; o < 8; o <<= 1) {
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        const D l2 = __shfl_xor(lo[d], o);
+        const D h2 = __shfl_xor(hi[d], o);
+        lo[d] = l2 < lo[d] ? l2 : lo[d];
+        hi[d] = h2 > hi[d] ? h2 : hi[d];
+      }
+    }
+    if (sub == 0) {
+      BoxRec<D, DIM> b;
+#pragma unroll
+      for (int d = 0; d < PadDim<DIM>::value; ++d) {
+        b.lo[d] = d < DIM ? lo[d < DIM ? d : 0] : D(0);
+        b.hi[d] = d < DIM ? hi[d < DIM ? d : 0] : D(0);
+      }
+      box[i] = b;
+    }
+  }
+}
+
+// Boxes of the inner nodes of level l from their children's boxes (leaves
+// are done by leaf_box_kernel); run deepest level first.
+template <typename D, int DIM>
+static __global__ __launch_bounds__(kLevelBlock) void box_level_dev_kernel(const NodeRec* __restrict__ nodes,
                                                                BoxRec<D, DIM>* __restrict__ box,
-                                                               int32_t* __restrict__ leafof,
                                                                const int32_t* __restrict__ lev_s,
                                                                const int32_t* __restrict__ lev_e,
                                                                int l) {
@@ -104,31 +158,7 @@ static __global__ __launch_bounds__(kLevelBlock) void box_level_dev_kernel(const
       b.hi[d] = d < DIM ? -pos_inf<D>() : D(0);
     }
     if (nr.child < 0) {
-      // four independent loads in flight per step
-      int32_t p = nr.a;
-      for (; p + 4 <= nr.b; p += 4) {
-        PointRec<D, DIM> r[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) r[u] = rec[p + u];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          leafof[p + u] = (int32_t)i;
-#pragma unroll
-          for (int d = 0; d < DIM; ++d) {
-            b.lo[d] = r[u].c[d] < b.lo[d] ? r[u].c[d] : b.lo[d];
-            b.hi[d] = r[u].c[d] > b.hi[d] ? r[u].c[d] : b.hi[d];
-          }
-        }
-      }
-      for (; p < nr.b; ++p) {
-        leafof[p] = (int32_t)i;
-        const PointRec<D, DIM> r = rec[p];
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) {
-          b.lo[d] = r.c[d] < b.lo[d] ? r.c[d] : b.lo[d];
-          b.hi[d] = r.c[d] > b.hi[d] ? r.c[d] : b.hi[d];
-        }
-      }
+      continue;  // leaf boxes come from leaf_box_kernel
     } else {
       const BoxRec<D, DIM> x = box[nr.child];
       const BoxRec<D, DIM> y = box[nr.child + 1];

@@ -343,6 +343,33 @@ def test_rnear_self_and_points(gpu, dtype):
             assert np.array_equal(np.sort(res[i]), idx[off[i]:off[i + 1]]), i
 
 
+def test_rnear_ragged_waves_and_heavy_queries(gpu):
+    """Query counts that leave the last wave of the count and fill passes
+    partly filled (the passes take 32 and 16 queries per wave), below and
+    above the size where queries are Morton-ordered, on a cloud whose dense
+    cluster gives thousands of hits per query next to sparse ones (the fill
+    launches its heaviest waves first)."""
+    rng = np.random.default_rng(23)
+    pts = np.concatenate([rng.random((12_000, 3)),
+                          0.5 + 0.03 * rng.random((6_000, 3))]).astype(np.float32)
+    kd = _kd()
+    tree = kd._build(pts)
+    ot = O.OracleTree(pts)
+    r = float(np.float32(0.04))
+    for m in [1, 15, 17, 31, 33, 4096 + 13]:
+        q = np.concatenate([rng.random((m // 2, 3)), 0.5 + 0.03 * rng.random((m - m // 2, 3))])
+        res = kd._query(tree, q, k=None, dmax=r)
+        off, idx = ot.rnear(q.astype(ot.dtype), r)
+        assert len(res) == m
+        for i in range(m):
+            assert np.array_equal(np.sort(res[i]), idx[off[i]:off[i + 1]]), (m, i)
+    res = kd._query(tree, None, k=None, dmax=r)
+    off, idx = ot.rnear(None, r)
+    assert int(off[-1]) > 50 * len(pts)
+    for i in range(0, len(res), 3):
+        assert np.array_equal(np.sort(res[i]), idx[off[i]:off[i + 1]]), i
+
+
 def test_rnear_large_radius_whole_subtrees(gpu):
     rng = np.random.default_rng(22)
     pts = rng.random((5000, 3)).astype(np.float32)

@@ -9,10 +9,11 @@
 //     (box lower bound <= that lane's bound, below);
 //   * a leaf's points are loaded 64 at a time, one per lane, into LDS, and
 //     filtered against the bounding box of the queries whose ball reaches
-//     the leaf and the largest bound among them; each surviving point is read back by every lane (one
-//     broadcast LDS read), every lane computes its own distance, and a lane
-//     whose key is below its bound appends the key to its own buffer in LDS
-//     (branch-free: the slot past the last key always holds +inf);
+//     the leaf and the largest bound among them; each surviving point is
+//     read back by every lane (one broadcast LDS read), every lane computes
+//     its own distance, and a lane whose key is below its bound appends the
+//     key to its own buffer in LDS (branch-free: every lane writes its key to
+//     the slot past its last one, and only a taken key advances the count);
 //   * when any lane's buffer is full, every lane sorts its buffer (bitonic
 //     network) and merges it into its sorted top-k (half-cleaner against the
 //     reversed buffer, then a bitonic merge), all as 64-bit key min/max.
@@ -43,9 +44,17 @@ template <int K> constexpr int coop_buf_bytes() {
              ? coop_slots<K>() * 64 * 8
              : ((K + 1) * 64 * 4 + 15) / 16 * 16;
 }
+// A leaf point staged in LDS: the original index first, so that a lane's
+// read of it leaves the index in an even register, the low half of the key
+// (distance bits, index) it builds.
+template <int DIM>
+struct alignas(16) CoopRec {
+  int32_t orig;
+  float c[DIM];
+};
 // LDS bytes for one wave: the buffer area, then one leaf chunk of records.
 template <int K, int DIM> constexpr int coop_wave_bytes() {
-  return coop_buf_bytes<K>() + 64 * (int)sizeof(PointRec<float, DIM>);
+  return coop_buf_bytes<K>() + 64 * (int)sizeof(CoopRec<DIM>);
 }
 // After the search: slot j of this lane's neighbour positions (j <= K).
 __device__ __forceinline__ int32_t* coop_positions(uint64_t* wave_lds) {
@@ -157,7 +166,7 @@ struct CoopSearch {
   const TreeView<float, DIM>& tv;
   TopK<float, K, false>& st;
   uint64_t* lbuf;                // buffers: slot s of lane l at lbuf[s * 64 + l]
-  PointRec<float, DIM>* chunk;   // the loaded leaf chunk, record l from lane l
+  CoopRec<DIM>* chunk;           // the loaded leaf chunk's surviving points
   const float* q;
   int lane;
   int cnt;                       // keys in this lane's buffer
@@ -168,7 +177,7 @@ struct CoopSearch {
   __device__ __forceinline__ CoopSearch(const TreeView<float, DIM>& tv_, TopK<float, K, false>& st_,
                                         uint64_t* wave_lds, const float* q_)
       : tv(tv_), st(st_), lbuf(wave_lds),
-        chunk((PointRec<float, DIM>*)((char*)wave_lds + coop_buf_bytes<K>())),
+        chunk((CoopRec<DIM>*)((char*)wave_lds + coop_buf_bytes<K>())),
         q(q_), lane(threadIdx.x & 63), cnt(0), capkey(0), bkey(0), bd(-1.0f) {}
 
   __device__ __forceinline__ void refresh(bool part) {
@@ -182,8 +191,8 @@ struct CoopSearch {
     uint64_t b[M];
 #pragma unroll
     for (int s = 0; s < M; ++s) {
-      b[s] = lbuf[s * 64 + lane];
-      lbuf[s * 64 + lane] = kKeyEmpty;
+      const uint64_t v = lbuf[s * 64 + lane];
+      b[s] = s < cnt ? v : kKeyEmpty;
     }
     cnt = 0;
     key_sort<M>(b);
@@ -257,8 +266,6 @@ __device__ __forceinline__ void knn_coop_self(const TreeView<float, DIM>& tv, bo
                                               TopK<float, K, false>& st, uint64_t* wave_lds) {
   CoopSearch<DIM, K> cs(tv, st, wave_lds, q);
   constexpr int M = CoopSearch<DIM, K>::M;
-#pragma unroll
-  for (int s = 0; s < M; ++s) cs.lbuf[s * 64 + cs.lane] = kKeyEmpty;
   int32_t root = 0;
   if (tv.nroots > 1 && active) root = root_by_range<DIM>(tv, pos);
   // cap: the largest distance of the k points nearest to pos in tree order
@@ -325,13 +332,14 @@ __device__ __forceinline__ void knn_coop_self(const TreeView<float, DIM>& tv, bo
       bool full = false;
       if (ci < ncand) {
         do {
-          const PointRec<float, DIM> cr = cs.chunk[ci];
+          const CoopRec<DIM> cr = cs.chunk[ci];
           ++ci;
           const float dd = dist2<float, DIM>(q, cr.c);
           const uint64_t key = ((uint64_t)__float_as_uint(dd) << 32) | (uint32_t)cr.orig;
           // the own point has the lane's own index (unique in its cloud)
           const bool take = (key < cs.bkey) & !(excl_self & (cr.orig == self_orig));
-          cs.lbuf[cs.cnt * 64 + cs.lane] = take ? key : kKeyEmpty;
+          // written whether taken or not: slot cnt is past the last key
+          cs.lbuf[cs.cnt * 64 + cs.lane] = key;
           cs.cnt += take ? 1 : 0;
           full = __ballot(cs.cnt == M) != 0ull;
         } while ((ci < ncand) & !full);
@@ -346,7 +354,13 @@ __device__ __forceinline__ void knn_coop_self(const TreeView<float, DIM>& tv, bo
         if (valid) rec = tv.pts[p];
         const bool pass = valid && box_min_dist2<float, DIM>(lqb, rec.c) <= rmax;
         const uint64_t pm = __ballot(pass);
-        if (pass) cs.chunk[rank_below(pm)] = rec;
+        if (pass) {
+          CoopRec<DIM> cr;
+          cr.orig = rec.orig;
+#pragma unroll
+          for (int d = 0; d < DIM; ++d) cr.c[d] = rec.c[d];
+          cs.chunk[rank_below(pm)] = cr;
+        }
         ncand = __popcll(pm);
         ci = 0;
         c0 += 64;

@@ -535,7 +535,7 @@ namespace {
 
 template <typename D, int DIM>
 __global__ __launch_bounds__(kBlock) void query_morton_kernel(const D* __restrict__ qpts, int64_t m,
-                                                              BuildInfo<D> frame,
+                                                              BuildInfo<D> frame, int shift,
                                                               uint64_t* __restrict__ keys,
                                                               uint32_t* __restrict__ vals) {
   const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -543,7 +543,7 @@ __global__ __launch_bounds__(kBlock) void query_morton_kernel(const D* __restric
   D c[DIM];
 #pragma unroll
   for (int d = 0; d < DIM; ++d) c[d] = qpts[t * DIM + d];
-  keys[t] = morton_code<D, DIM>(c, frame);
+  keys[t] = morton_code<D, DIM>(c, frame) >> shift;
   vals[t] = (uint32_t)t;
 }
 
@@ -564,17 +564,31 @@ __global__ __launch_bounds__(kBlock) void query_position_kernel(const int32_t* _
 // tree position. Below this many queries the sort costs more than it saves.
 constexpr int64_t kOrderMin = 4096;
 
+// Point queries are ordered on the top kQueryOrderBits bits of their Morton
+// code (16 bits per axis in 3-D): finer cells than any wave's extent, in
+// fewer radix passes than the full code.
+constexpr int kQueryOrderBits = 48;
+
 template <typename D>
 void order_queries(const pptk_tree* t, PreparedQueries& pq, hipStream_t s) {
   const int64_t m = pq.m;
   if (m < kOrderMin || pq.mode == kQueryAll) return;
+  OrderCache& oc = t->order_cache;
+  {
+    std::lock_guard<std::mutex> lk(oc.mu);
+    if (oc.order && oc.src == pq.src && oc.m == m && oc.mode == pq.mode) {
+      pq.order_buf = oc.order;
+      pq.order = pq.order_buf->as<const int32_t>();
+      return;
+    }
+  }
   DevBuf keys((size_t)m * 8);
-  pq.order_buf.reset((size_t)m * 4);
+  auto buf = std::make_shared<DevBuf>((size_t)m * 4);
   int bits = 32;
   if (pq.mode == kQueryList) {
     query_position_kernel<<<grid_of(m), kBlock, 0, s>>>(
         pq.dlist, t->fwd.as<const int32_t>(), m, t->n_input, keys.as<uint64_t>(),
-        pq.order_buf.as<uint32_t>());
+        buf->as<uint32_t>());
     PK_LAUNCH_CHECK();
   } else {
     BuildInfo<D> frame;
@@ -587,17 +601,25 @@ void order_queries(const pptk_tree* t, PreparedQueries& pq, hipStream_t s) {
     }
     auto launch = [&](auto dimc) {
       constexpr int DIM = decltype(dimc)::value;
+      constexpr int full = DIM * MortonBits<DIM>::value;
+      constexpr int shift = full > kQueryOrderBits ? full - kQueryOrderBits : 0;
       query_morton_kernel<D, DIM><<<grid_of(m), kBlock, 0, s>>>(
-          (const D*)pq.dpts, m, frame, keys.as<uint64_t>(), pq.order_buf.as<uint32_t>());
+          (const D*)pq.dpts, m, frame, shift, keys.as<uint64_t>(), buf->as<uint32_t>());
       PK_LAUNCH_CHECK();
-      bits = DIM * MortonBits<DIM>::value;
+      bits = full - shift;
     };
     if (t->dim == 2) launch(std::integral_constant<int, 2>());
     else if (t->dim == 3) launch(std::integral_constant<int, 3>());
     else launch(std::integral_constant<int, 4>());
   }
-  radix_sort_pairs(keys.as<uint64_t>(), pq.order_buf.as<uint32_t>(), m, bits, s);
-  pq.order = pq.order_buf.as<const int32_t>();
+  radix_sort_pairs(keys.as<uint64_t>(), buf->as<uint32_t>(), m, bits, s);  // synchronises s
+  pq.order_buf = buf;
+  pq.order = buf->as<const int32_t>();
+  std::lock_guard<std::mutex> lk(oc.mu);
+  oc.src = pq.src;
+  oc.m = m;
+  oc.mode = pq.mode;
+  oc.order = std::move(buf);
 }
 
 }  // namespace
@@ -624,6 +646,7 @@ void prepare_queries(const pptk_tree* t, const pptk_queries* q, hipStream_t s, P
       }
       stage_input(pq.list, q->indices, (size_t)q->m * 4, q->mem, s);
       pq.dlist = (const int32_t*)pq.list.ptr;
+      pq.src = q->indices;
       break;
     }
     case PPTK_Q_POINTS: {
@@ -633,6 +656,7 @@ void prepare_queries(const pptk_tree* t, const pptk_queries* q, hipStream_t s, P
       pq.mode = kQueryPoints;
       if (q->m < 0) fail(PPTK_E_VALUE, "negative query count");
       pq.m = pq.nrows = q->m;
+      pq.src = q->points;
       if (q->m > 0 && !q->points) fail(PPTK_E_VALUE, "null query points");
       if (!dtype_valid(q->dtype)) fail(PPTK_E_TYPE, "unsupported query dtype %d", (int)q->dtype);
       const size_t cnt = (size_t)q->m * t->dim;

@@ -12,9 +12,10 @@ struct PreparedQueries {
   const int32_t* dlist = nullptr;
   DevBuf conv;
   const void* dpts = nullptr;
+  const void* src = nullptr;   // the caller's query array (order cache key)
   // Processing order (thread t serves query order[t]) so that neighbouring
   // lanes get nearby queries; null = input order.
-  DevBuf order_buf;
+  std::shared_ptr<DevBuf> order_buf;
   const int32_t* order = nullptr;
 };
 

@@ -1,6 +1,24 @@
 #pragma once
 #include "common.hpp"
 
+#include <mutex>
+
+namespace pk {
+// The processing order of the last ordered query set served by a tree, kept
+// for the next call on the same query array with the same count (an r-near
+// count pass and then its fill pass). Any permutation of the queries gives
+// the same results, so a stale entry (same array, new contents) only costs
+// locality. An entry is published after the stream that made it has
+// synchronised.
+struct OrderCache {
+  std::mutex mu;
+  const void* src = nullptr;
+  int64_t m = -1;
+  int mode = -1;
+  std::shared_ptr<DevBuf> order;
+};
+}  // namespace pk
+
 // Host-side tree handle (opaque `pptk_tree` of the C ABI).
 struct pptk_tree {
   static constexpr uint64_t kMagic = 0x70707462676b6474ull;
@@ -25,6 +43,7 @@ struct pptk_tree {
   pk::DevBuf box;   // BoxRec<D,dim>[nn]
   pk::DevBuf node;  // NodeRec[nn]
   pk::DevBuf leafof;  // int32[n_valid]: leaf node holding each tree position
+  mutable pk::OrderCache order_cache;
 };
 
 namespace pk {

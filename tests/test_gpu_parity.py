@@ -1,6 +1,8 @@
 """GPU parity: the HIP path (through the drop-in modules and the C ABI)
 against the oracle on the same seeded inputs. Index work is compared
 bit-exactly; normals by angle against FP64."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -368,6 +370,57 @@ def test_rnear_ragged_waves_and_heavy_queries(gpu):
     assert int(off[-1]) > 50 * len(pts)
     for i in range(0, len(res), 3):
         assert np.array_equal(np.sort(res[i]), idx[off[i]:off[i + 1]]), i
+
+
+def test_query_order_reuse_same_array_new_contents(gpu):
+    """A tree keeps the processing order of its last query array for the next
+    call on the same array and count. Rewriting the array in place between
+    calls leaves a stale order behind: results must not change (only the
+    order of work does)."""
+    import torch
+    from pptk_amd import capi
+    rng = np.random.default_rng(24)
+    pts = rng.random((40_000, 3)).astype(np.float32)
+    t = capi.Tree.build(pts)
+    ot = O.OracleTree(pts)
+    r = float(np.float32(0.03))
+    m = 6000
+    dev = torch.device("cuda:0")
+    q_d = torch.empty((m, 3), dtype=torch.float32, device=dev)
+    off_d = torch.empty(m + 1, dtype=torch.int64, device=dev)
+    qd, _ = capi.make_queries(dev_ptr=q_d.data_ptr(), m=m, kind=capi.Q_POINTS, dtype=np.float32)
+    L = capi.lib()
+    for rep in range(3):
+        q = rng.random((m, 3)).astype(np.float32) * (1.0 if rep != 1 else 0.25)
+        q_d.copy_(torch.from_numpy(q))
+        torch.cuda.synchronize()
+        total = ctypes.c_int64()
+        capi.check(L.pptk_rnear_count(ctypes.c_void_p(t.handle), ctypes.byref(qd), r,
+                                      ctypes.c_void_p(off_d.data_ptr()), ctypes.byref(total),
+                                      capi.MEM_DEVICE, None))
+        idx_d = torch.empty(max(1, total.value), dtype=torch.int32, device=dev)
+        capi.check(L.pptk_rnear_fill(ctypes.c_void_p(t.handle), ctypes.byref(qd), r,
+                                     ctypes.c_void_p(off_d.data_ptr()),
+                                     ctypes.c_void_p(idx_d.data_ptr()), capi.MEM_DEVICE, None))
+        torch.cuda.synchronize()
+        off, idx = off_d.cpu().numpy(), idx_d.cpu().numpy()
+        woff, widx = ot.rnear(q, r)
+        assert np.array_equal(np.diff(off), np.diff(woff)), rep
+        for i in range(0, m, 5):
+            assert np.array_equal(np.sort(idx[off[i]:off[i + 1]]), widx[woff[i]:woff[i + 1]]), (rep, i)
+    # index queries: one host array rewritten between kNN calls
+    qi = rng.integers(0, len(pts), 5000).astype(np.int32)
+    qs = capi.Queries()
+    qs.kind, qs.m, qs.indices, qs.mem = capi.Q_INDICES, len(qi), qi.ctypes.data, capi.MEM_HOST
+    gi = np.empty((len(qi), 8), np.int32)
+    gc = np.empty(len(qi), np.int32)
+    for rep in range(3):
+        capi.check(L.pptk_knn(ctypes.c_void_p(t.handle), ctypes.byref(qs), 8, float("inf"),
+                              ctypes.c_void_p(gi.ctypes.data), ctypes.c_void_p(gc.ctypes.data),
+                              None, capi.MEM_HOST, None))
+        wi, wc = ot.knn(qi, 8)
+        assert np.array_equal(gi, wi) and np.array_equal(gc, wc), rep
+        qi[:] = rng.integers(0, len(pts), len(qi))
 
 
 def test_rnear_large_radius_whole_subtrees(gpu):

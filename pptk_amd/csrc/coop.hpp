@@ -188,11 +188,14 @@ struct CoopSearch {
 
   // Sorts every lane's buffer into its top-k and empties the buffers.
   __device__ __forceinline__ void flush(bool part) {
+    // a slot at or past cnt gets distance bits +inf: it sorts after every
+    // real key and, if it reaches the top-k, reads as no neighbour
     uint64_t b[M];
 #pragma unroll
     for (int s = 0; s < M; ++s) {
       const uint64_t v = lbuf[s * 64 + lane];
-      b[s] = s < cnt ? v : kKeyEmpty;
+      const uint32_t hi = s < cnt ? (uint32_t)(v >> 32) : 0x7f800000u;
+      b[s] = ((uint64_t)hi << 32) | (uint32_t)v;
     }
     cnt = 0;
     key_sort<M>(b);

@@ -486,10 +486,12 @@ __global__ __launch_bounds__(kBlock) void rnear_packet_kernel(TreeView<D, DIM> t
 
 // Sort key of each r-near fill wave (pk consecutive queries): the
 // complement of its output count, so that an ascending sort puts the waves
-// with the most output first.
+// with the most output first. Thread t serves row order[t] (or t); for
+// self queries of every point (row_of_pos set) it serves row_of_pos[t].
 __global__ __launch_bounds__(kBlock) void rnear_wave_cost_kernel(const int32_t* __restrict__ order,
+                                                                 const int32_t* __restrict__ row_of_pos,
                                                                  const int64_t* __restrict__ offsets,
-                                                                 int64_t m, int pk,
+                                                                 int64_t m, int64_t nrows, int pk,
                                                                  uint64_t* __restrict__ keys,
                                                                  uint32_t* __restrict__ vals) {
   const int64_t w = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -498,8 +500,8 @@ __global__ __launch_bounds__(kBlock) void rnear_wave_cost_kernel(const int32_t* 
   for (int l = 0; l < pk; ++l) {
     const int64_t t = w * pk + l;
     if (t >= m) break;
-    const int64_t row = order ? order[t] : t;
-    c += offsets[row + 1] - offsets[row];
+    const int64_t row = row_of_pos ? row_of_pos[t] : (order ? order[t] : t);
+    if (row >= 0 && row < nrows) c += offsets[row + 1] - offsets[row];
   }
   keys[w] = 0xffffffffull - (uint64_t)(c < 0xffffffffll ? c : 0xffffffffll);
   vals[w] = (uint32_t)w;
@@ -828,7 +830,8 @@ void rnear_fill_run(const pptk_tree* t, const pptk_queries* qs, double dmax, con
       wkeys.reset((size_t)nw * 8);
       wvals.reset((size_t)nw * 4);
       rnear_wave_cost_kernel<<<grid_of(nw), kBlock, 0, s>>>(
-          pq.order, (const int64_t*)doff.ptr, pq.m, pk, wkeys.as<uint64_t>(), wvals.as<uint32_t>());
+          pq.order, pq.mode == kQueryAll ? tv.orig : nullptr, (const int64_t*)doff.ptr, pq.m,
+          nrows, pk, wkeys.as<uint64_t>(), wvals.as<uint32_t>());
       PK_LAUNCH_CHECK();
       radix_sort_pairs(wkeys.as<uint64_t>(), wvals.as<uint32_t>(), nw, 32, s);
       worder = wvals.as<uint32_t>();

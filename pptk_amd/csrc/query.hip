@@ -184,6 +184,7 @@ __global__ __launch_bounds__(kBlock) void rnear_kernel(TreeView<D, DIM> tv, int 
                                                        int64_t m, int64_t nrows, D r2,
                                                        int32_t* __restrict__ counts,
                                                        const int64_t* __restrict__ offsets,
+                                                       int64_t total,
                                                        int32_t* __restrict__ out) {
   const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const int lane = threadIdx.x & 63;
@@ -199,8 +200,12 @@ __global__ __launch_bounds__(kBlock) void rnear_kernel(TreeView<D, DIM> tv, int 
   bool active = has_row && ok;
   int64_t base = 0, cap = 0;
   if (FILL && active) {
+    // writes stay inside [offsets[row], offsets[row + 1]) clamped to [0, total)
+    int64_t e = offsets[row + 1];
     base = offsets[row];
-    cap = offsets[row + 1] - base;
+    base = base < 0 ? 0 : (base > total ? total : base);
+    e = e < base ? base : (e > total ? total : e);
+    cap = e - base;
   }
   int32_t cnt = 0;  // outputs so far for this lane's query
   int32_t node = 0;
@@ -362,6 +367,7 @@ __global__ __launch_bounds__(kBlock) void rnear_packet_kernel(TreeView<D, DIM> t
                                                               const uint32_t* __restrict__ wave_order,
                                                               int32_t* __restrict__ counts,
                                                               const int64_t* __restrict__ offsets,
+                                                              int64_t total,
                                                               int32_t* __restrict__ out) {
   const int lane = threadIdx.x & 63;
   int64_t w = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
@@ -391,8 +397,17 @@ __global__ __launch_bounds__(kBlock) void rnear_packet_kernel(TreeView<D, DIM> t
     qr.orig = excl;
     wq[lane] = qr;
     if constexpr (FILL) {
-      ww[2 * lane] = active ? offsets[row] : 0;
-      ww[2 * lane + 1] = active ? offsets[row + 1] : 0;
+      // a lane writes only inside [offsets[row], offsets[row + 1]) clamped
+      // to the output's [0, total)
+      int64_t a = 0, e = 0;
+      if (active) {
+        a = offsets[row];
+        e = offsets[row + 1];
+        a = a < 0 ? 0 : (a > total ? total : a);
+        e = e < a ? a : (e > total ? total : e);
+      }
+      ww[2 * lane] = a;
+      ww[2 * lane + 1] = e;
     }
   }
   int32_t cnt = 0;                  // count: outputs so far for this lane's query
@@ -501,7 +516,10 @@ __global__ __launch_bounds__(kBlock) void rnear_wave_cost_kernel(const int32_t* 
     const int64_t t = w * pk + l;
     if (t >= m) break;
     const int64_t row = row_of_pos ? row_of_pos[t] : (order ? order[t] : t);
-    if (row >= 0 && row < nrows) c += offsets[row + 1] - offsets[row];
+    if (row >= 0 && row < nrows) {
+      const int64_t n = offsets[row + 1] - offsets[row];
+      c += n > 0 ? n : 0;
+    }
   }
   keys[w] = 0xffffffffull - (uint64_t)(c < 0xffffffffll ? c : 0xffffffffll);
   vals[w] = (uint32_t)w;
@@ -773,15 +791,15 @@ void rnear_count_run(const pptk_tree* t, const pptk_queries* qs, double dmax, in
     if (tv.nroots == 1 && pq.mode == kQueryPoints)
       rnear_packet_kernel<D, DIM, false, false><<<grid, kBlock, 0, s>>>(
           tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.order, pq.m, nrows, r2, pk, nullptr,
-          counts.as<int32_t>(), nullptr, nullptr);
+          counts.as<int32_t>(), nullptr, 0, nullptr);
     else if (tv.nroots == 1)
       rnear_packet_kernel<D, DIM, false, true><<<grid, kBlock, 0, s>>>(
           tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.order, pq.m, nrows, r2, pk, nullptr,
-          counts.as<int32_t>(), nullptr, nullptr);
+          counts.as<int32_t>(), nullptr, 0, nullptr);
     else
       rnear_kernel<D, DIM, false><<<grid_of(pq.m), kBlock, 0, s>>>(
           tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.order, pq.m, nrows, r2,
-          counts.as<int32_t>(), nullptr, nullptr);
+          counts.as<int32_t>(), nullptr, 0, nullptr);
     PK_LAUNCH_CHECK();
   }
   int64_t* doff = (int64_t*)oo.dptr;
@@ -839,15 +857,15 @@ void rnear_fill_run(const pptk_tree* t, const pptk_queries* qs, double dmax, con
     if (tv.nroots == 1 && pq.mode == kQueryPoints)
       rnear_packet_kernel<D, DIM, true, false><<<grid, kBlock, 0, s>>>(
           tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.order, pq.m, nrows, r2, pk, worder,
-          nullptr, (const int64_t*)doff.ptr, (int32_t*)oi.dptr);
+          nullptr, (const int64_t*)doff.ptr, total, (int32_t*)oi.dptr);
     else if (tv.nroots == 1)
       rnear_packet_kernel<D, DIM, true, true><<<grid, kBlock, 0, s>>>(
           tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.order, pq.m, nrows, r2, pk, worder,
-          nullptr, (const int64_t*)doff.ptr, (int32_t*)oi.dptr);
+          nullptr, (const int64_t*)doff.ptr, total, (int32_t*)oi.dptr);
     else
       rnear_kernel<D, DIM, true><<<grid_of(pq.m), kBlock, 0, s>>>(
           tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.order, pq.m, nrows, r2, nullptr,
-          (const int64_t*)doff.ptr, (int32_t*)oi.dptr);
+          (const int64_t*)doff.ptr, total, (int32_t*)oi.dptr);
     PK_LAUNCH_CHECK();
   }
   oi.finish(s);

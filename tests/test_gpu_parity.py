@@ -423,6 +423,47 @@ def test_query_order_reuse_same_array_new_contents(gpu):
         qi[:] = rng.integers(0, len(pts), len(qi))
 
 
+@pytest.mark.parametrize("kind", ["points", "self"])
+def test_rnear_fill_stays_inside_output_with_bad_offsets(gpu, kind):
+    """Offsets that do not come from the count pass (too large, decreasing)
+    must not make the fill write outside [0, offsets[m])."""
+    import torch
+    from pptk_amd import capi
+    rng = np.random.default_rng(25)
+    pts = rng.random((20_000, 3)).astype(np.float32)
+    t = capi.Tree.build(pts)
+    r = float(np.float32(0.05))
+    dev = torch.device("cuda:0")
+    if kind == "points":
+        q = torch.from_numpy(rng.random((5000, 3)).astype(np.float32)).to(dev)
+        qd, _ = capi.make_queries(dev_ptr=q.data_ptr(), m=5000, kind=capi.Q_POINTS,
+                                  dtype=np.float32)
+        m = 5000
+    else:
+        qd, _ = capi.make_queries(None, mem=capi.MEM_DEVICE)
+        qd.kind = capi.Q_ALL
+        m = len(pts)
+    L = capi.lib()
+    off_d = torch.empty(m + 1, dtype=torch.int64, device=dev)
+    total = ctypes.c_int64()
+    capi.check(L.pptk_rnear_count(ctypes.c_void_p(t.handle), ctypes.byref(qd), r,
+                                  ctypes.c_void_p(off_d.data_ptr()), ctypes.byref(total),
+                                  capi.MEM_DEVICE, None))
+    good = off_d.cpu().numpy()
+    bad = good.copy()
+    bad[1:-1] = bad[1:-1][::-1]              # decreasing inside
+    bad[m // 2] = good[-1] + 100_000         # past the end
+    bad[-1] = good[-1] // 2                  # a smaller total
+    off_d.copy_(torch.from_numpy(bad))
+    guard = 1 << 16
+    out = torch.full((int(bad[-1]) + guard,), -7, dtype=torch.int32, device=dev)
+    capi.check(L.pptk_rnear_fill(ctypes.c_void_p(t.handle), ctypes.byref(qd), r,
+                                 ctypes.c_void_p(off_d.data_ptr()),
+                                 ctypes.c_void_p(out.data_ptr()), capi.MEM_DEVICE, None))
+    torch.cuda.synchronize()
+    assert bool((out[int(bad[-1]):] == -7).all())
+
+
 def test_rnear_large_radius_whole_subtrees(gpu):
     rng = np.random.default_rng(22)
     pts = rng.random((5000, 3)).astype(np.float32)

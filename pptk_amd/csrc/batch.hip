@@ -374,10 +374,12 @@ __global__ __launch_bounds__(kBlock) void roots_kernel(const uint32_t* __restric
 // with the point merged in at its (0, own index) rank, truncated to nk: the
 // same set, summed in the same order, as a separate search would give.
 template <typename D, int DIM, int K, int PREC, bool NORM>
-__global__ __launch_bounds__(kBlock) void batch_query_kernel(TreeView<D, DIM> tv, int64_t N,
+__global__ __launch_bounds__(kBlock, K <= 16 ? 8 : 1) void batch_query_kernel(
+                                                             TreeView<D, DIM> tv, int64_t N,
                                                              const uint32_t* __restrict__ vtotal,
                                                              const int32_t* __restrict__ gorig,
                                                              int k, int kk, int nk, D r2,
+                                                             int wave_words,
                                                              int32_t* __restrict__ out_idx,
                                                              int32_t* __restrict__ out_cnt,
                                                              D* __restrict__ out_nrm,
@@ -408,7 +410,7 @@ __global__ __launch_bounds__(kBlock) void batch_query_kernel(TreeView<D, DIM> tv
   uint64_t* wl = nullptr;
   if constexpr (kCoop) {
     extern __shared__ uint64_t coop_lds[];
-    wl = coop_lds + (threadIdx.x >> 6) * (coop_wave_bytes<K, DIM>() / 8);
+    wl = coop_lds + (threadIdx.x >> 6) * wave_words;
     knn_coop_self<DIM, K>(tv, valid, (int32_t)t, self, q, r2, kk, true, st, wl);
     if (t >= N) return;
   } else {
@@ -461,7 +463,7 @@ __global__ __launch_bounds__(kBlock) void batch_query_kernel(TreeView<D, DIM> tv
       if constexpr (kCoop) {
         // staged in this lane's LDS buffer (free after the search), so the
         // keys are dead before the covariance's two passes
-        int32_t* pl = coop_positions(wl);
+        int32_t* pl = coop_positions<DIM>(wl);
         int m = 0;
         merged([&](int32_t p) {
           pl[m * 64] = p;
@@ -575,11 +577,16 @@ void enqueue_run(pptk_batch* b, const D* pts, int32_t* out_idx, int32_t* out_cnt
       constexpr int K = decltype(kc)::value;
       constexpr int P = decltype(pc)::value;
       constexpr bool NM = decltype(nc)::value;
+      // per wave: the leaf chunk, then the key buffers or, after the
+      // search, the nk neighbour positions of each lane
+      const int pos_bytes = (std::max(b->nk, 0) * 64 * 4 + 15) / 16 * 16;
+      const int wave_bytes = coop_chunk_bytes<DIM>() +
+                             std::max(coop_buf_bytes<K, 0>(), pos_bytes);
       const size_t lds = std::is_same<D, float>::value
-                             ? (size_t)(b->qblock / 64) * coop_wave_bytes<K, DIM>() : 0;
+                             ? (size_t)(b->qblock / 64) * wave_bytes : 0;
       batch_query_kernel<D, DIM, K, P, NM><<<grid, b->qblock, lds, s>>>(
-          tv, b->N, vtotal, gorig, b->k, b->kk, b->nk, r2, out_idx, out_cnt, out_nrm, out_eval,
-          nfb);
+          tv, b->N, vtotal, gorig, b->k, b->kk, b->nk, r2, wave_bytes / 8, out_idx, out_cnt,
+          out_nrm, out_eval, nfb);
     };
     auto by_k = [&](auto pc, auto nc) {
       if (b->kk <= 4) go(std::integral_constant<int, 4>{}, pc, nc);

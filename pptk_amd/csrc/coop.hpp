@@ -32,18 +32,8 @@
 
 namespace pk {
 
-constexpr uint64_t kKeyEmpty = (0x7f800000ull << 32) | 0xffffffffull;
-
 // Buffer slots per lane for a top-k of K: 8 (4 for K = 4).
 template <int K> constexpr int coop_slots() { return K < 8 ? K : 8; }
-// LDS bytes of one wave's buffer area: the lanes' key buffers during the
-// search; afterwards the same bytes hold up to K + 1 neighbour positions per
-// lane (coop_positions).
-template <int K> constexpr int coop_buf_bytes() {
-  return coop_slots<K>() * 64 * 8 > ((K + 1) * 64 * 4 + 15) / 16 * 16
-             ? coop_slots<K>() * 64 * 8
-             : ((K + 1) * 64 * 4 + 15) / 16 * 16;
-}
 // A leaf point staged in LDS: the original index first, so that a lane's
 // read of it leaves the index in an even register, the low half of the key
 // (distance bits, index) it builds.
@@ -52,13 +42,22 @@ struct alignas(16) CoopRec {
   int32_t orig;
   float c[DIM];
 };
-// LDS bytes for one wave: the buffer area, then one leaf chunk of records.
-template <int K, int DIM> constexpr int coop_wave_bytes() {
-  return coop_buf_bytes<K>() + 64 * (int)sizeof(CoopRec<DIM>);
+// One wave's LDS: a leaf chunk of 64 records, then the buffer area. The
+// buffer area holds the lanes' key buffers during the search; afterwards it
+// may hold up to NPOS neighbour positions per lane (coop_positions).
+template <int DIM> constexpr int coop_chunk_bytes() { return 64 * (int)sizeof(CoopRec<DIM>); }
+template <int K, int NPOS> constexpr int coop_buf_bytes() {
+  return coop_slots<K>() * 64 * 8 > (NPOS * 64 * 4 + 15) / 16 * 16
+             ? coop_slots<K>() * 64 * 8
+             : (NPOS * 64 * 4 + 15) / 16 * 16;
 }
-// After the search: slot j of this lane's neighbour positions (j <= K).
+template <int K, int DIM, int NPOS> constexpr int coop_wave_bytes() {
+  return coop_chunk_bytes<DIM>() + coop_buf_bytes<K, NPOS>();
+}
+// After the search: slot j of this lane's neighbour positions (j < NPOS).
+template <int DIM>
 __device__ __forceinline__ int32_t* coop_positions(uint64_t* wave_lds) {
-  return (int32_t*)wave_lds + (threadIdx.x & 63);
+  return (int32_t*)((char*)wave_lds + coop_chunk_bytes<DIM>()) + (threadIdx.x & 63);
 }
 
 // Compare-exchange of two keys (a <= b after). One asm block, b updated in
@@ -176,8 +175,8 @@ struct CoopSearch {
 
   __device__ __forceinline__ CoopSearch(const TreeView<float, DIM>& tv_, TopK<float, K, false>& st_,
                                         uint64_t* wave_lds, const float* q_)
-      : tv(tv_), st(st_), lbuf(wave_lds),
-        chunk((CoopRec<DIM>*)((char*)wave_lds + coop_buf_bytes<K>())),
+      : tv(tv_), st(st_), lbuf((uint64_t*)((char*)wave_lds + coop_chunk_bytes<DIM>())),
+        chunk((CoopRec<DIM>*)wave_lds),
         q(q_), lane(threadIdx.x & 63), cnt(0), capkey(0), bkey(0), bd(-1.0f) {}
 
   __device__ __forceinline__ void refresh(bool part) {
@@ -261,7 +260,7 @@ struct CoopSearch {
 // self_orig), done by the whole wave. Every lane of the wave must call it
 // (inactive lanes with active = false). excl_self: the point itself is not a
 // neighbour (kNN self queries); otherwise it is one (normals). st must be freshly init(k);
-// wave_lds is this wave's coop_wave_bytes<K, DIM>() of LDS.
+// wave_lds is this wave's coop_wave_bytes<K, DIM, NPOS>() of LDS (any NPOS).
 template <int DIM, int K>
 __device__ __forceinline__ void knn_coop_self(const TreeView<float, DIM>& tv, bool active,
                                               int32_t pos, int32_t self_orig, const float* q,

@@ -111,13 +111,13 @@ __global__ __launch_bounds__(kBlock) void normals_knn_kernel(TreeView<D, 3> tv,
 // normals_knn_kernel with the wave-cooperative search (coop.hpp). Lanes past
 // the end stay in the wave as inactive lanes: the search needs every lane.
 template <int K, int PREC, int NE>
-__global__ __launch_bounds__(kBlock) void normals_coop_kernel(TreeView<float, 3> tv, int64_t m,
+__global__ __launch_bounds__(kBlock, K <= 16 ? 8 : 1) void normals_coop_kernel(TreeView<float, 3> tv, int64_t m,
                                                               int k, float r2,
                                                               float* __restrict__ evecs,
                                                               float* __restrict__ evals,
                                                               int32_t* __restrict__ sizes,
                                                               unsigned long long* __restrict__ nfb) {
-  __shared__ uint64_t lds[kBlock / 64 * coop_wave_bytes<K, 3>() / 8];
+  __shared__ uint64_t lds[kBlock / 64 * coop_wave_bytes<K, 3, K>() / 8];
   const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   float q[3] = {0.0f, 0.0f, 0.0f};
   int64_t row = -1;
@@ -129,12 +129,12 @@ __global__ __launch_bounds__(kBlock) void normals_coop_kernel(TreeView<float, 3>
   }
   TopK<float, K, false> st;
   st.init(k);
-  uint64_t* wl = lds + (threadIdx.x >> 6) * (coop_wave_bytes<K, 3>() / 8);
+  uint64_t* wl = lds + (threadIdx.x >> 6) * (coop_wave_bytes<K, 3, K>() / 8);
   knn_coop_self<3, K>(tv, row >= 0, (int32_t)t, (int32_t)row, q, r2, k, false, st, wl);
   if (row < 0) return;
   // The neighbours' tree positions, in key order, go to this lane's LDS
   // buffer: the keys are dead before the covariance's two passes.
-  int32_t* pos = coop_positions(wl);
+  int32_t* pos = coop_positions<3>(wl);
   int n = 0;
 #pragma unroll
   for (int s = 0; s < K; ++s) {

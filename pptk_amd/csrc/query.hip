@@ -65,12 +65,12 @@ __global__ __launch_bounds__(kBlock) void knn_kernel(TreeView<D, DIM> tv, int mo
 // wave-cooperative search per wave (coop.hpp). Lanes past the end stay in
 // the wave as inactive lanes: the search needs every lane.
 template <int DIM, int K>
-__global__ __launch_bounds__(kBlock) void knn_coop_kernel(TreeView<float, DIM> tv, int64_t m,
+__global__ __launch_bounds__(kBlock, K <= 16 ? 8 : 1) void knn_coop_kernel(TreeView<float, DIM> tv, int64_t m,
                                                           int64_t nrows, int k, float r2,
                                                           int32_t* __restrict__ out_idx,
                                                           int32_t* __restrict__ out_cnt,
                                                           float* __restrict__ out_dist) {
-  __shared__ uint64_t lds[kBlock / 64 * coop_wave_bytes<K, DIM>() / 8];
+  __shared__ uint64_t lds[kBlock / 64 * coop_wave_bytes<K, DIM, 0>() / 8];
   const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   float q[DIM];
 #pragma unroll
@@ -87,7 +87,7 @@ __global__ __launch_bounds__(kBlock) void knn_coop_kernel(TreeView<float, DIM> t
   TopK<float, K, false> st;
   st.init(k);
   knn_coop_self<DIM, K>(tv, row >= 0, (int32_t)t, (int32_t)row, q, r2, k, true, st,
-                        lds + (threadIdx.x >> 6) * (coop_wave_bytes<K, DIM>() / 8));
+                        lds + (threadIdx.x >> 6) * (coop_wave_bytes<K, DIM, 0>() / 8));
   if (row < 0) return;
   int32_t c = 0;
   int32_t* orow = out_idx + row * k;

@@ -374,7 +374,7 @@ __global__ __launch_bounds__(kBlock) void roots_kernel(const uint32_t* __restric
 // with the point merged in at its (0, own index) rank, truncated to nk: the
 // same set, summed in the same order, as a separate search would give.
 template <typename D, int DIM, int K, int PREC, bool NORM>
-__global__ __launch_bounds__(kBlock, K <= 16 ? 8 : 1) void batch_query_kernel(
+__global__ __launch_bounds__(kBlock) void batch_query_kernel(
                                                              TreeView<D, DIM> tv, int64_t N,
                                                              const uint32_t* __restrict__ vtotal,
                                                              const int32_t* __restrict__ gorig,

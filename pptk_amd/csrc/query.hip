@@ -469,21 +469,33 @@ __global__ __launch_bounds__(kBlock) void rnear_packet_kernel(TreeView<D, DIM> t
             const bool valid = p < nr.b;
             PointRec<D, DIM> rec{};
             if (valid) rec = tv.pts[p];
+            // hit masks are ballots of the bare distance test, combined with
+            // the chunk's valid lanes (and the own position) as scalar masks
+            const uint64_t vmask = __ballot(valid);
             uint64_t sm = strm;
             while (sm) {
               const int L = __builtin_ctzll(sm);
               sm &= sm - 1ull;
               const PointRec<D, DIM> qL = wq[L];
-              bool pass = valid & (dist2<D, DIM>(qL.c, rec.c) < r2);
-              if constexpr (EXCL) pass &= p != qL.orig;
-              const uint64_t hits = __ballot(pass);
+              const bool in = dist2<D, DIM>(qL.c, rec.c) < r2;
+              uint64_t hits = __ballot(in) & vmask;
+              int32_t exL = -1;
+              if constexpr (EXCL) {
+                exL = __builtin_amdgcn_readfirstlane(qL.orig);
+                const uint32_t dl = (uint32_t)(exL - s0);
+                hits &= dl < 64u ? ~(1ull << dl) : ~0ull;
+              }
               if constexpr (FILL) {
                 const int64_t wL = ww[2 * L];
                 const int64_t o = wL + rank_below(hits);
-                if (pass & (o < ww[2 * L + 1])) out[o] = rec.orig;
+                if (in & valid & (p != exL) & (o < ww[2 * L + 1])) out[o] = rec.orig;
                 if (lane == 0) ww[2 * L] = wL + __popcll(hits);
               } else {
-                if (lane == L) cnt += __popcll(hits);
+                // this count belongs to lane L: read, add and write it back
+                // as a scalar
+                const int32_t c = __builtin_amdgcn_readlane(cnt, L) + __popcll(hits);
+                asm("s_mov_b32 m0, %2\n\tv_writelane_b32 %0, %1, m0" : "+v"(cnt) : "s"(c), "s"(L)
+                    : "m0");
               }
             }
           }

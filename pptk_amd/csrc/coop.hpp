@@ -169,7 +169,7 @@ struct CoopSearch {
   const float* q;
   int lane;
   int cnt;                       // keys in this lane's buffer
-  uint64_t capkey;               // the cap as a key: (cap bits, 0xffffffff)
+  uint32_t capbits;              // the cap's bits: as a key, (capbits, 0xffffffff)
   uint64_t bkey;                 // keys below bkey are taken; 0 = not searching
   float bd;                      // bkey's distance, -1 when not searching
 
@@ -177,10 +177,10 @@ struct CoopSearch {
                                         uint64_t* wave_lds, const float* q_)
       : tv(tv_), st(st_), lbuf((uint64_t*)((char*)wave_lds + coop_chunk_bytes<DIM>())),
         chunk((CoopRec<DIM>*)wave_lds),
-        q(q_), lane(threadIdx.x & 63), cnt(0), capkey(0), bkey(0), bd(-1.0f) {}
+        q(q_), lane(threadIdx.x & 63), cnt(0), capbits(0), bkey(0), bd(-1.0f) {}
 
   __device__ __forceinline__ void refresh(bool part) {
-    const uint64_t b = key_min(st.key[K - 1], capkey);
+    const uint64_t b = key_min(st.key[K - 1], ((uint64_t)capbits << 32) | 0xffffffffull);
     bkey = part ? b : 0ull;
     bd = part ? key_dist(b) : -1.0f;
   }
@@ -258,9 +258,11 @@ struct CoopSearch {
 
 // k-nearest search for the tree point at position pos (record index
 // self_orig), done by the whole wave. Every lane of the wave must call it
-// (inactive lanes with active = false). excl_self: the point itself is not a
-// neighbour (kNN self queries); otherwise it is one (normals). st must be freshly init(k);
-// wave_lds is this wave's coop_wave_bytes<K, DIM, NPOS>() of LDS (any NPOS).
+// (inactive lanes with active = false), and lane l must pass the wave's first
+// position + l as pos (consecutive positions, inactive lanes included).
+// excl_self: the point itself is not a neighbour (kNN self queries);
+// otherwise it is one (normals). st must be freshly init(k); wave_lds is this
+// wave's coop_wave_bytes<K, DIM, NPOS>() of LDS (any NPOS).
 template <int DIM, int K>
 __device__ __forceinline__ void knn_coop_self(const TreeView<float, DIM>& tv, bool active,
                                               int32_t pos, int32_t self_orig, const float* q,
@@ -268,6 +270,8 @@ __device__ __forceinline__ void knn_coop_self(const TreeView<float, DIM>& tv, bo
                                               TopK<float, K, false>& st, uint64_t* wave_lds) {
   CoopSearch<DIM, K> cs(tv, st, wave_lds, q);
   constexpr int M = CoopSearch<DIM, K>::M;
+  // the wave's first position: lane l's position is pos0 + l
+  const int32_t pos0 = __builtin_amdgcn_readfirstlane(pos - (int32_t)(threadIdx.x & 63));
   int32_t root = 0;
   if (tv.nroots > 1 && active) root = root_by_range<DIM>(tv, pos);
   // cap: the largest distance of the k points nearest to pos in tree order
@@ -296,7 +300,7 @@ __device__ __forceinline__ void knn_coop_self(const TreeView<float, DIM>& tv, bo
   {
     const float r2b = below(r2);
     capd = cap < r2b ? cap : r2b;
-    cs.capkey = ((uint64_t)__float_as_uint(capd) << 32) | 0xffffffffull;
+    cs.capbits = __float_as_uint(capd);
   }
   // Lanes whose cap is far above the wave's median (binary exponent more than
   // 3 above the median's: about 8x in squared distance) walk in a segment of
@@ -379,7 +383,7 @@ __device__ __forceinline__ void knn_coop_self(const TreeView<float, DIM>& tv, bo
         const uint64_t pm = __ballot(part);
         todo &= ~pm;
         cs.refresh(part);
-        mid = (rl_i32(pos, __builtin_ctzll(pm)) + rl_i32(pos, 63 - __builtin_clzll(pm))) >> 1;
+        mid = pos0 + ((__builtin_ctzll(pm) + (63 - __builtin_clzll(pm))) >> 1);
         node = R;
         depth = 0;
         flo = fhi = 0;

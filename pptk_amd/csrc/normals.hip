@@ -120,18 +120,20 @@ __global__ __launch_bounds__(kBlock, K <= 16 ? 8 : 1) void normals_coop_kernel(T
   __shared__ uint64_t lds[kBlock / 64 * coop_wave_bytes<K, 3, K>() / 8];
   const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   float q[3] = {0.0f, 0.0f, 0.0f};
-  int64_t row = -1;
+  bool active = false;
   if (t < m && !(tv.n_dev && t >= *tv.n_dev)) {
     const PointRec<float, 3> r = tv.pts[t];
-    row = r.orig;
+    active = true;
 #pragma unroll
     for (int d = 0; d < 3; ++d) q[d] = r.c[d];
   }
   TopK<float, K, false> st;
   st.init(k);
   uint64_t* wl = lds + (threadIdx.x >> 6) * (coop_wave_bytes<K, 3, K>() / 8);
-  knn_coop_self<3, K>(tv, row >= 0, (int32_t)t, (int32_t)row, q, r2, k, false, st, wl);
-  if (row < 0) return;
+  knn_coop_self<3, K>(tv, active, (int32_t)t, -1, q, r2, k, false, st, wl);
+  if (!active) return;
+  // the original row is read again rather than held through the search
+  const int32_t row = tv.orig[t];
   // The neighbours' tree positions, in key order, go to this lane's LDS
   // buffer: the keys are dead before the covariance's two passes.
   int32_t* pos = coop_positions<3>(wl);

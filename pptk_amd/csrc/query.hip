@@ -75,7 +75,7 @@ __global__ __launch_bounds__(kBlock, K <= 16 ? 8 : 1) void knn_coop_kernel(TreeV
   float q[DIM];
 #pragma unroll
   for (int d = 0; d < DIM; ++d) q[d] = 0.0f;
-  int64_t row = -1;
+  int32_t row = -1;   // original row (int32, like every original index)
   if (t < m && !(tv.n_dev && t >= *tv.n_dev)) {
     const PointRec<float, DIM> r = tv.pts[t];
     if (r.orig < nrows) {
@@ -86,18 +86,18 @@ __global__ __launch_bounds__(kBlock, K <= 16 ? 8 : 1) void knn_coop_kernel(TreeV
   }
   TopK<float, K, false> st;
   st.init(k);
-  knn_coop_self<DIM, K>(tv, row >= 0, (int32_t)t, (int32_t)row, q, r2, k, true, st,
+  knn_coop_self<DIM, K>(tv, row >= 0, (int32_t)t, row, q, r2, k, true, st,
                         lds + (threadIdx.x >> 6) * (coop_wave_bytes<K, DIM, 0>() / 8));
   if (row < 0) return;
   int32_t c = 0;
-  int32_t* orow = out_idx + row * k;
+  int32_t* orow = out_idx + (int64_t)row * k;
 #pragma unroll
   for (int s = 0; s < K; ++s) {
     if (s >= K - k) {
       const float ds = st.dist(s);
       const bool f = ds < pos_inf<float>();
       orow[s - (K - k)] = f ? st.idx(s) : -1;
-      if (out_dist) out_dist[row * k + (s - (K - k))] = f ? ds : pos_inf<float>();
+      if (out_dist) out_dist[(int64_t)row * k + (s - (K - k))] = f ? ds : pos_inf<float>();
       c += f ? 1 : 0;
     }
   }

@@ -118,7 +118,11 @@ __global__ __launch_bounds__(kBlock, K <= 16 ? 8 : 1) void normals_coop_kernel(T
                                                               int32_t* __restrict__ sizes,
                                                               unsigned long long* __restrict__ nfb) {
   __shared__ uint64_t lds[kBlock / 64 * coop_wave_bytes<K, 3, K>() / 8];
-  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  // the wave's first position and LDS area are scalars; a lane's position
+  // is recomputed after the search rather than held through it
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t t0 = (int64_t)blockIdx.x * kBlock + wave * 64;
+  int64_t t = t0 + __lane_id();
   float q[3] = {0.0f, 0.0f, 0.0f};
   bool active = false;
   if (t < m && !(tv.n_dev && t >= *tv.n_dev)) {
@@ -129,10 +133,10 @@ __global__ __launch_bounds__(kBlock, K <= 16 ? 8 : 1) void normals_coop_kernel(T
   }
   TopK<float, K, false> st;
   st.init(k);
-  uint64_t* wl = lds + (threadIdx.x >> 6) * (coop_wave_bytes<K, 3, K>() / 8);
+  uint64_t* wl = lds + wave * (coop_wave_bytes<K, 3, K>() / 8);
   knn_coop_self<3, K>(tv, active, (int32_t)t, -1, q, r2, k, false, st, wl);
   if (!active) return;
-  // the original row is read again rather than held through the search
+  t = t0 + __lane_id();
   const int32_t row = tv.orig[t];
   // The neighbours' tree positions, in key order, go to this lane's LDS
   // buffer: the keys are dead before the covariance's two passes.

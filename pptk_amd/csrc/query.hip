@@ -65,7 +65,7 @@ __global__ __launch_bounds__(kBlock) void knn_kernel(TreeView<D, DIM> tv, int mo
 // wave-cooperative search per wave (coop.hpp). Lanes past the end stay in
 // the wave as inactive lanes: the search needs every lane.
 template <int DIM, int K>
-__global__ __launch_bounds__(kBlock, K <= 16 ? 8 : 1) void knn_coop_kernel(TreeView<float, DIM> tv, int64_t m,
+__global__ __launch_bounds__(kBlock, K <= 16 && DIM <= 3 ? 8 : 1) void knn_coop_kernel(TreeView<float, DIM> tv, int64_t m,
                                                           int64_t nrows, int k, float r2,
                                                           int32_t* __restrict__ out_idx,
                                                           int32_t* __restrict__ out_cnt,
@@ -443,7 +443,7 @@ __global__ __launch_bounds__(kBlock) void rnear_packet_kernel(TreeView<D, DIM> t
               if constexpr (EXCL) {
                 const int32_t exL = wq[L].orig;
                 // the own position is skipped: later points move down one slot
-                o -= (exL >= nr.a) & (p > exL) ? 1 : 0;
+                o -= ((exL >= nr.a) & (p > exL)) ? 1 : 0;
                 w &= p != exL;
               }
               if (w & (o < eL)) out[o] = og;
@@ -452,13 +452,13 @@ __global__ __launch_bounds__(kBlock) void rnear_packet_kernel(TreeView<D, DIM> t
         }
         if (inside) {
           int32_t size = nr.b - nr.a;
-          if constexpr (EXCL) size -= (excl >= nr.a) & (excl < nr.b) ? 1 : 0;
+          if constexpr (EXCL) size -= ((excl >= nr.a) & (excl < nr.b)) ? 1 : 0;
           ww[2 * lane] += size;
         }
       } else {
         if (inside) {
           int32_t size = nr.b - nr.a;
-          if constexpr (EXCL) size -= (excl >= nr.a) & (excl < nr.b) ? 1 : 0;
+          if constexpr (EXCL) size -= ((excl >= nr.a) & (excl < nr.b)) ? 1 : 0;
           cnt += size;
         }
       }
@@ -491,11 +491,7 @@ __global__ __launch_bounds__(kBlock) void rnear_packet_kernel(TreeView<D, DIM> t
                 if (in & valid & (p != exL) & (o < ww[2 * L + 1])) out[o] = rec.orig;
                 if (lane == 0) ww[2 * L] = wL + __popcll(hits);
               } else {
-                // this count belongs to lane L: read, add and write it back
-                // as a scalar
-                const int32_t c = __builtin_amdgcn_readlane(cnt, L) + __popcll(hits);
-                asm("s_mov_b32 m0, %2\n\tv_writelane_b32 %0, %1, m0" : "+v"(cnt) : "s"(c), "s"(L)
-                    : "m0");
+                cnt += lane == L ? (int32_t)__popcll(hits) : 0;
               }
             }
           }

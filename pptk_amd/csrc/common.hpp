@@ -193,10 +193,13 @@ __device__ __forceinline__ D box_min_dist2(const BoxRec<D, DIM>& b, const D* q) 
   D s = D(0);
 #pragma unroll
   for (int i = 0; i < DIM; ++i) {
-    D dmin = b.lo[i] - q[i];
-    D dmax = b.hi[i] - q[i];
-    D v = dmin > D(0) ? dmin * dmin : (dmax < D(0) ? dmax * dmax : D(0));
-    s = s + v;
+    const D dmin = b.lo[i] - q[i];
+    const D dmax = b.hi[i] - q[i];
+    // the gap on the side of the box the query lies beyond (lo <= hi, so at
+    // most one side), squared once: the same value as squaring that side's
+    // difference, selected without a branch
+    const D e = dmin > D(0) ? dmin : (dmax < D(0) ? dmax : D(0));
+    s = s + e * e;
   }
   return s;
 }
@@ -208,10 +211,10 @@ __device__ __forceinline__ D box_max_dist2(const BoxRec<D, DIM>& b, const D* q) 
   D s = D(0);
 #pragma unroll
   for (int i = 0; i < DIM; ++i) {
-    D dmin = b.lo[i] - q[i];
-    D dmax = b.hi[i] - q[i];
-    D a = dmin * dmin, c = dmax * dmax;
-    s = s + (a > c ? a : c);
+    const D dmin = b.lo[i] - q[i];
+    const D dmax = b.hi[i] - q[i];
+    // the larger square (both are products, never NaN for finite boxes)
+    s = s + fmax(dmin * dmin, dmax * dmax);
   }
   return s;
 }

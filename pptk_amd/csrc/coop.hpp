@@ -358,8 +358,9 @@ __device__ __forceinline__ void knn_coop_self(const TreeView<float, DIM>& tv, bo
         const bool valid = p < cend;
         PointRec<float, DIM> rec{};
         if (valid) rec = tv.pts[p];
-        const bool pass = valid && box_min_dist2<float, DIM>(lqb, rec.c) <= rmax;
-        const uint64_t pm = __ballot(pass);
+        const bool near = box_min_dist2<float, DIM>(lqb, rec.c) <= rmax;
+        const bool pass = valid & near;
+        const uint64_t pm = __ballot(valid) & __ballot(near);
         if (pass) {
           CoopRec<DIM> cr;
           cr.orig = rec.orig;

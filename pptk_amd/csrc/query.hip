@@ -363,7 +363,7 @@ __global__ __launch_bounds__(kBlock) void rnear_packet_kernel(TreeView<D, DIM> t
                                                               const D* __restrict__ qpts,
                                                               const int32_t* __restrict__ order,
                                                               int64_t m, int64_t nrows, D r2,
-                                                              int pk,
+                                                              int pk, int64_t nw,
                                                               const uint32_t* __restrict__ wave_order,
                                                               int32_t* __restrict__ counts,
                                                               const int64_t* __restrict__ offsets,
@@ -371,7 +371,7 @@ __global__ __launch_bounds__(kBlock) void rnear_packet_kernel(TreeView<D, DIM> t
                                                               int32_t* __restrict__ out) {
   const int lane = threadIdx.x & 63;
   int64_t w = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
-  if (wave_order && w < (m + pk - 1) / pk) w = wave_order[w];
+  if (wave_order && w < nw) w = wave_order[w];
   const int64_t t = w * pk + lane;
   D q[DIM];
 #pragma unroll
@@ -412,18 +412,25 @@ __global__ __launch_bounds__(kBlock) void rnear_packet_kernel(TreeView<D, DIM> t
   }
   int32_t cnt = 0;                  // count: outputs so far for this lane's query
   int32_t done_end = INT32_MIN;     // nodes starting before this are taken
-  int32_t node = __ballot(active) ? 0 : -1;
+  // wave masks are ballots of bare comparisons combined as scalars (a ballot
+  // of a combined per-lane predicate costs two more vector instructions)
+  const uint64_t actm = __ballot(active);
+  int32_t node = actm ? 0 : -1;
   while (node >= 0) {
     node = __builtin_amdgcn_readfirstlane(node);
     const NodeRec nr = tv.node[node];
     const BoxRec<D, DIM> bx = tv.box[node];
-    const bool hit = active & (nr.a >= done_end) & (box_min_dist2<D, DIM>(bx, q) < r2);
+    const bool c_new = nr.a >= done_end;
+    const bool c_min = box_min_dist2<D, DIM>(bx, q) < r2;
+    const bool hit = active & c_new & c_min;
+    const uint64_t hitm = __ballot(c_new) & __ballot(c_min) & actm;
     bool descend = false;
-    if (__ballot(hit)) {
-      const bool inside = hit & (box_max_dist2<D, DIM>(bx, q) < r2);
-      const bool strad = hit & !inside;
-      const uint64_t insm = __ballot(inside);
-      const uint64_t strm = __ballot(strad);
+    if (hitm) {
+      const bool c_max = box_max_dist2<D, DIM>(bx, q) < r2;
+      const bool inside = hit & c_max;
+      const uint64_t cmaxm = __ballot(c_max);
+      const uint64_t insm = hitm & cmaxm;
+      const uint64_t strm = hitm & ~cmaxm;
       if (inside) done_end = nr.b;
       if constexpr (FILL) {
         if (insm) {
@@ -798,12 +805,12 @@ void rnear_count_run(const pptk_tree* t, const pptk_queries* qs, double dmax, in
     const unsigned grid = (unsigned)std::max<int64_t>(1, ceil_div(ceil_div(pq.m, pk), kBlock / 64));
     if (tv.nroots == 1 && pq.mode == kQueryPoints)
       rnear_packet_kernel<D, DIM, false, false><<<grid, kBlock, 0, s>>>(
-          tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.order, pq.m, nrows, r2, pk, nullptr,
-          counts.as<int32_t>(), nullptr, 0, nullptr);
+          tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.order, pq.m, nrows, r2, pk,
+          ceil_div(pq.m, pk), nullptr, counts.as<int32_t>(), nullptr, 0, nullptr);
     else if (tv.nroots == 1)
       rnear_packet_kernel<D, DIM, false, true><<<grid, kBlock, 0, s>>>(
-          tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.order, pq.m, nrows, r2, pk, nullptr,
-          counts.as<int32_t>(), nullptr, 0, nullptr);
+          tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.order, pq.m, nrows, r2, pk,
+          ceil_div(pq.m, pk), nullptr, counts.as<int32_t>(), nullptr, 0, nullptr);
     else
       rnear_kernel<D, DIM, false><<<grid_of(pq.m), kBlock, 0, s>>>(
           tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.order, pq.m, nrows, r2,
@@ -864,11 +871,11 @@ void rnear_fill_run(const pptk_tree* t, const pptk_queries* qs, double dmax, con
     }
     if (tv.nroots == 1 && pq.mode == kQueryPoints)
       rnear_packet_kernel<D, DIM, true, false><<<grid, kBlock, 0, s>>>(
-          tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.order, pq.m, nrows, r2, pk, worder,
+          tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.order, pq.m, nrows, r2, pk, nw, worder,
           nullptr, (const int64_t*)doff.ptr, total, (int32_t*)oi.dptr);
     else if (tv.nroots == 1)
       rnear_packet_kernel<D, DIM, true, true><<<grid, kBlock, 0, s>>>(
-          tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.order, pq.m, nrows, r2, pk, worder,
+          tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.order, pq.m, nrows, r2, pk, nw, worder,
           nullptr, (const int64_t*)doff.ptr, total, (int32_t*)oi.dptr);
     else
       rnear_kernel<D, DIM, true><<<grid_of(pq.m), kBlock, 0, s>>>(

@@ -103,31 +103,38 @@ __device__ __forceinline__ void key_merge(uint64_t (&a)[K], const uint64_t (&b)[
   }
 }
 
-// Wave-wide max / min, returned as a wave-uniform (scalar) value: xor
-// swizzles inside each 32-lane half, then the two halves' lane 0.
-template <int XOR>
-__device__ __forceinline__ float swz_xor(float x) {
-  return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(x), 0x1f | (XOR << 10)));
+// Floats as integers of the same order (no NaN): the magnitude bits of a
+// negative value are flipped. The map is its own inverse.
+__device__ __forceinline__ int32_t float_ord(float x) {
+  const int32_t b = __float_as_int(x);
+  return b ^ ((b >> 31) & 0x7fffffff);
 }
+__device__ __forceinline__ float ord_float(int32_t k) {
+  return __int_as_float(k ^ ((k >> 31) & 0x7fffffff));
+}
+// Wave maximum / minimum of x (finite or infinite, never NaN), as a scalar:
+// integer max/min of the order-preserving map, within each row of 16 lanes by
+// DPP quad permutations and row rotations (each folded into the max/min),
+// then across the four rows' first lanes as scalars.
 __device__ __forceinline__ float wave_max_f(float x) {
-  float y;
-  y = swz_xor<1>(x); x = x > y ? x : y;
-  y = swz_xor<2>(x); x = x > y ? x : y;
-  y = swz_xor<4>(x); x = x > y ? x : y;
-  y = swz_xor<8>(x); x = x > y ? x : y;
-  y = swz_xor<16>(x); x = x > y ? x : y;
-  const float a = rl(x, 0), b = rl(x, 32);
-  return a > b ? a : b;
+  int32_t k = float_ord(x);
+  k = max(k, __builtin_amdgcn_mov_dpp(k, 0xb1, 0xf, 0xf, true));    // quad_perm [1,0,3,2]
+  k = max(k, __builtin_amdgcn_mov_dpp(k, 0x4e, 0xf, 0xf, true));    // quad_perm [2,3,0,1]
+  k = max(k, __builtin_amdgcn_mov_dpp(k, 0x124, 0xf, 0xf, true));   // row_ror:4
+  k = max(k, __builtin_amdgcn_mov_dpp(k, 0x128, 0xf, 0xf, true));   // row_ror:8
+  const int32_t r0 = __builtin_amdgcn_readlane(k, 0), r1 = __builtin_amdgcn_readlane(k, 16);
+  const int32_t r2 = __builtin_amdgcn_readlane(k, 32), r3 = __builtin_amdgcn_readlane(k, 48);
+  return ord_float(max(max(r0, r1), max(r2, r3)));
 }
 __device__ __forceinline__ float wave_min_f(float x) {
-  float y;
-  y = swz_xor<1>(x); x = x < y ? x : y;
-  y = swz_xor<2>(x); x = x < y ? x : y;
-  y = swz_xor<4>(x); x = x < y ? x : y;
-  y = swz_xor<8>(x); x = x < y ? x : y;
-  y = swz_xor<16>(x); x = x < y ? x : y;
-  const float a = rl(x, 0), b = rl(x, 32);
-  return a < b ? a : b;
+  int32_t k = float_ord(x);
+  k = min(k, __builtin_amdgcn_mov_dpp(k, 0xb1, 0xf, 0xf, true));
+  k = min(k, __builtin_amdgcn_mov_dpp(k, 0x4e, 0xf, 0xf, true));
+  k = min(k, __builtin_amdgcn_mov_dpp(k, 0x124, 0xf, 0xf, true));
+  k = min(k, __builtin_amdgcn_mov_dpp(k, 0x128, 0xf, 0xf, true));
+  const int32_t r0 = __builtin_amdgcn_readlane(k, 0), r1 = __builtin_amdgcn_readlane(k, 16);
+  const int32_t r2 = __builtin_amdgcn_readlane(k, 32), r3 = __builtin_amdgcn_readlane(k, 48);
+  return ord_float(min(min(r0, r1), min(r2, r3)));
 }
 
 // Number of set bits of mask in the lanes below this one (v_mbcnt).

@@ -394,11 +394,12 @@ __device__ void knn_search_from_leaf(const TreeView<D, DIM>& tv, int32_t pos, co
 // when the walk has left the tree (a root has parent -1).
 template <typename D, int DIM>
 __device__ __forceinline__ int32_t next_in_order(const TreeView<D, DIM>& tv, int32_t node) {
+  // roots are exactly the nodes [0, nroots) (parent -1); every other node
+  // is in a child pair that starts at an odd id
   for (;;) {
-    const int32_t par = tv.node[node].parent;
-    if (par < 0) return -1;
+    if (node < tv.nroots) return -1;
     if (node & 1) return node + 1;  // left child: its sibling is next
-    node = par;
+    node = tv.node[node].parent;
   }
 }
 

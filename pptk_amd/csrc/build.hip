@@ -335,9 +335,11 @@ void build_impl(pptk_tree* t, const T* dpts, int64_t n, hipStream_t s) {
   const int64_t cap = std::max<int64_t>(1, 2 * nv - 1);
   DevBuf nodes((size_t)cap * sizeof(NodeRec));
   // Level table: every split launch reads its range from device memory and
-  // publishes the next level, so the host reads it back only once per
-  // kLevelBatch levels to see whether the tree is complete. Depth is bounded
-  // by the key bits plus 31 halvings of equal keys.
+  // publishes the next level, so the host reads it back only once per batch
+  // of levels to see whether the tree is complete: first after the balanced
+  // depth log2(nv / B) plus 12 (Morton splits are uneven), then every
+  // kLevelBatch levels. A launch past the last level only reads an empty
+  // range. Depth is bounded by the key bits plus 31 halvings of equal keys.
   constexpr int kLevelBatch = 8;
   const int max_levels = DIM * MortonBits<DIM>::value + 33;
   DevBuf lev((size_t)(2 * (max_levels + 1)) * 4);
@@ -381,8 +383,11 @@ void build_impl(pptk_tree* t, const T* dpts, int64_t n, hipStream_t s) {
       (unsigned)std::min<int64_t>(1024, std::max<int64_t>(1, ceil_div(nv, (int64_t)B * kBlock)));
   int nlev = 1;  // levels [0, nlev) hold nodes
   if (nv > 0) {
-    for (int l0 = 0;; l0 += kLevelBatch) {
-      const int l1 = std::min(l0 + kLevelBatch, max_levels);
+    int balanced = 0;
+    while (((int64_t)B << balanced) < nv) ++balanced;
+    for (int l0 = 0;;) {
+      const int l1 = std::min(l0 == 0 ? std::max(kLevelBatch, balanced + 12) : l0 + kLevelBatch,
+                              max_levels);
       for (int l = l0; l < l1; ++l) {
         split_level_dev_kernel<<<split_grid, kLevelBlock, 0, s>>>(
             keys.as<uint64_t>(), nodes.as<NodeRec>(), lev_s, lev_e, done.as<uint32_t>(), l,
@@ -394,6 +399,7 @@ void build_impl(pptk_tree* t, const T* dpts, int64_t n, hipStream_t s) {
       PK_HIP(hipMemcpyAsync(&next[1], lev_e + l1, 4, hipMemcpyDeviceToHost, s));
       PK_HIP(hipStreamSynchronize(s));
       if (next[1] <= next[0] || l1 >= max_levels) break;
+      l0 = l1;
     }
   }
   LevelState st;

@@ -18,26 +18,34 @@
 namespace pk {
 
 template <typename F> __device__ __forceinline__ F fabs_(F x) { return x < F(0) ? -x : x; }
-__device__ __forceinline__ float sqrt_(float x) { return sqrtf(x); }
+// Rotation parameters in float use the hardware square root, reciprocal and
+// reciprocal square root (about 1 ulp each): a rotation's angle only steers
+// convergence, and c*c + s*s stays 1 to rounding, so the float solve keeps
+// its accuracy without IEEE division sequences. Double stays exact.
+__device__ __forceinline__ float sqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ double sqrt_(double x) { return sqrt(x); }
+__device__ __forceinline__ float rcp_(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double rcp_(double x) { return 1.0 / x; }
+__device__ __forceinline__ float rsq_(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ double rsq_(double x) { return 1.0 / sqrt(x); }
 
 template <typename F, int P, int Q>
 __device__ __forceinline__ void jacobi_rotate(F (&a)[3][3], F (&v)[3][3]) {
   constexpr int R = 3 - P - Q;
   const F apq = a[P][Q];
   if (apq == F(0)) return;
-  const F theta = (a[Q][Q] - a[P][P]) / (F(2) * apq);
+  const F theta = (a[Q][Q] - a[P][P]) * rcp_(F(2) * apq);
   const F at = fabs_(theta);
   F t;
   if (at > F(1e15)) {
-    t = F(0.5) / theta;
+    t = F(0.5) * rcp_(theta);
   } else {
-    t = F(1) / (at + sqrt_(theta * theta + F(1)));
+    t = rcp_(at + sqrt_(theta * theta + F(1)));
     if (theta < F(0)) t = -t;
   }
-  const F c = F(1) / sqrt_(t * t + F(1));
+  const F c = rsq_(t * t + F(1));
   const F s = t * c;
-  const F tau = s / (F(1) + c);
+  const F tau = s * rcp_(F(1) + c);
   a[P][P] = a[P][P] - t * apq;
   a[Q][Q] = a[Q][Q] + t * apq;
   a[P][Q] = F(0);

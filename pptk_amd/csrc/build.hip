@@ -445,12 +445,14 @@ std::set<const void*> g_trees;
 
 }  // namespace
 
-// Measured on MI355X (10M-point lidar scene, leaf sizes 32..128, k = 16):
-// kNN, r-near and normals are each fastest at 96 points per leaf (the step
-// is 7% faster than at 64, and 1% faster than at 128).
+// Measured on MI355X (10M-point lidar scene: build, kNN k = 16 of every
+// point, r-near of 1M points at 0.1 m, normals k = 16): the step takes
+// 16.0 / 15.4 / 14.8 / 14.6 / 14.5 / 14.3 ms at 64 / 80 / 96 / 112 / 128 /
+// 160 points per leaf and stays within noise up to 256. Point queries, which
+// scan leaves lane by lane, are 5% slower at 160 than at 96.
 int default_leaf_size(int dim, bool dbl, int64_t n) {
   (void)dim; (void)dbl; (void)n;
-  return 96;
+  return 160;
 }
 
 void register_tree(pptk_tree* t) {

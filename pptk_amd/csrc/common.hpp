@@ -193,12 +193,12 @@ __device__ __forceinline__ D box_min_dist2(const BoxRec<D, DIM>& b, const D* q) 
   D s = D(0);
 #pragma unroll
   for (int i = 0; i < DIM; ++i) {
-    const D dmin = b.lo[i] - q[i];
-    const D dmax = b.hi[i] - q[i];
     // the gap on the side of the box the query lies beyond (lo <= hi, so at
-    // most one side), squared once: the same value as squaring that side's
-    // difference, selected without a branch
-    const D e = dmin > D(0) ? dmin : (dmax < D(0) ? dmax : D(0));
+    // most one side is positive), squared once: q - hi is exactly -(hi - q),
+    // so the square is that side's. One three-way max for floats.
+    const D below = b.lo[i] - q[i];
+    const D above = q[i] - b.hi[i];
+    const D e = fmax(fmax(below, above), D(0));
     s = s + e * e;
   }
   return s;

@@ -423,7 +423,7 @@ __global__ __launch_bounds__(kBlock) void batch_query_kernel(
     for (int s = 0; s < K; ++s) {
       const int j = s - (K - kk);
       if (j >= 0 && j < k) {
-        const bool f = st.dist(s) < pos_inf<D>();
+        const bool f = st.found(s);
         orow[j] = f ? st.idx(s) : -1;
         c += f ? 1 : 0;
       }
@@ -435,7 +435,7 @@ __global__ __launch_bounds__(kBlock) void batch_query_kernel(
     const bool self_in = valid && D(0) < r2;
     int n = self_in ? 1 : 0;
 #pragma unroll
-    for (int s = 0; s < K; ++s) n += (s >= K - kk && st.dist(s) < pos_inf<D>()) ? 1 : 0;
+    for (int s = 0; s < K; ++s) n += (s >= K - kk && st.found(s)) ? 1 : 0;
     n = n < nk ? n : nk;
     if (n == 0) {
       write_nan<D, 1>(row, out_nrm, out_eval, nullptr);
@@ -446,7 +446,7 @@ __global__ __launch_bounds__(kBlock) void batch_query_kernel(
         int taken = 0;
 #pragma unroll
         for (int s = 0; s < K; ++s) {
-          if (s >= K - kk && st.dist(s) < pos_inf<D>()) {
+          if (s >= K - kk && st.found(s)) {
             if (pending && key_less(D(0), self, st.dist(s), st.idx(s))) {
               if (taken < nk) f((int32_t)t);
               ++taken;

@@ -126,6 +126,30 @@ __device__ __forceinline__ float wave_max_f(float x) {
   const int32_t r2 = __builtin_amdgcn_readlane(k, 32), r3 = __builtin_amdgcn_readlane(k, 48);
   return ord_float(max(max(r0, r1), max(r2, r3)));
 }
+// The same maxima and minima, carried across the rows by row broadcasts
+// (lane 15 into rows 1 and 3, then lane 31 into rows 2 and 3, each folded
+// into the max/min) and read from lane 63: fewer instructions, one more
+// vector register held until the read.
+__device__ __forceinline__ float wave_max_f_bcast(float x) {
+  int32_t k = float_ord(x);
+  k = max(k, __builtin_amdgcn_mov_dpp(k, 0xb1, 0xf, 0xf, true));
+  k = max(k, __builtin_amdgcn_mov_dpp(k, 0x4e, 0xf, 0xf, true));
+  k = max(k, __builtin_amdgcn_mov_dpp(k, 0x124, 0xf, 0xf, true));
+  k = max(k, __builtin_amdgcn_mov_dpp(k, 0x128, 0xf, 0xf, true));
+  k = max(k, __builtin_amdgcn_update_dpp(INT32_MIN, k, 0x142, 0xa, 0xf, false));  // row_bcast:15
+  k = max(k, __builtin_amdgcn_update_dpp(INT32_MIN, k, 0x143, 0xc, 0xf, false));  // row_bcast:31
+  return ord_float(__builtin_amdgcn_readlane(k, 63));
+}
+__device__ __forceinline__ float wave_min_f_bcast(float x) {
+  int32_t k = float_ord(x);
+  k = min(k, __builtin_amdgcn_mov_dpp(k, 0xb1, 0xf, 0xf, true));
+  k = min(k, __builtin_amdgcn_mov_dpp(k, 0x4e, 0xf, 0xf, true));
+  k = min(k, __builtin_amdgcn_mov_dpp(k, 0x124, 0xf, 0xf, true));
+  k = min(k, __builtin_amdgcn_mov_dpp(k, 0x128, 0xf, 0xf, true));
+  k = min(k, __builtin_amdgcn_update_dpp(INT32_MAX, k, 0x142, 0xa, 0xf, false));
+  k = min(k, __builtin_amdgcn_update_dpp(INT32_MAX, k, 0x143, 0xc, 0xf, false));
+  return ord_float(__builtin_amdgcn_readlane(k, 63));
+}
 __device__ __forceinline__ float wave_min_f(float x) {
   int32_t k = float_ord(x);
   k = min(k, __builtin_amdgcn_mov_dpp(k, 0xb1, 0xf, 0xf, true));
@@ -176,7 +200,6 @@ struct CoopSearch {
   const float* q;
   int lane;
   int cnt;                       // keys in this lane's buffer
-  uint32_t capbits;              // the cap's bits: as a key, (capbits, 0xffffffff)
   uint64_t bkey;                 // keys below bkey are taken; 0 = not searching
   float bd;                      // bkey's distance, -1 when not searching
 
@@ -184,10 +207,11 @@ struct CoopSearch {
                                         uint64_t* wave_lds, const float* q_)
       : tv(tv_), st(st_), lbuf((uint64_t*)((char*)wave_lds + coop_chunk_bytes<DIM>())),
         chunk((CoopRec<DIM>*)wave_lds),
-        q(q_), lane(threadIdx.x & 63), cnt(0), capbits(0), bkey(0), bd(-1.0f) {}
+        q(q_), lane(threadIdx.x & 63), cnt(0), bkey(0), bd(-1.0f) {}
 
   __device__ __forceinline__ void refresh(bool part) {
-    const uint64_t b = key_min(st.key[K - 1], ((uint64_t)capbits << 32) | 0xffffffffull);
+    // the cap is in the top-k: its empty slots start at the cap key
+    const uint64_t b = st.key[K - 1];
     bkey = part ? b : 0ull;
     bd = part ? key_dist(b) : -1.0f;
   }
@@ -269,8 +293,10 @@ struct CoopSearch {
 // position + l as pos (consecutive positions, inactive lanes included).
 // excl_self: the point itself is not a neighbour (kNN self queries);
 // otherwise it is one (normals). st must be freshly init(k); wave_lds is this
-// wave's coop_wave_bytes<K, DIM, NPOS>() of LDS (any NPOS).
-template <int DIM, int K>
+// wave's coop_wave_bytes<K, DIM, NPOS>() of LDS (any NPOS). ROWBCAST picks
+// the row-broadcast form of the reductions at each leaf (fewer instructions,
+// one more vector register).
+template <int DIM, int K, bool ROWBCAST = true>
 __device__ __forceinline__ void knn_coop_self(const TreeView<float, DIM>& tv, bool active,
                                               int32_t pos, int32_t self_orig, const float* q,
                                               float r2, int k, bool excl_self,
@@ -307,7 +333,12 @@ __device__ __forceinline__ void knn_coop_self(const TreeView<float, DIM>& tv, bo
   {
     const float r2b = below(r2);
     capd = cap < r2b ? cap : r2b;
-    cs.capbits = __float_as_uint(capd);
+    // the cap as a key, (cap, 0xffffffff), fills the empty slots: the top-k's
+    // k-th key is then the smaller of the k-th key found and the cap key, and
+    // a slot still holding it at the end reads as no neighbour (index -1)
+    const uint64_t capkey = ((uint64_t)__float_as_uint(capd) << 32) | 0xffffffffull;
+#pragma unroll
+    for (int s = 0; s < K; ++s) st.key[s] = key_min(st.key[s], capkey);
   }
   // Lanes whose cap is far above the wave's median (binary exponent more than
   // 3 above the median's: about 8x in squared distance) walk in a segment of
@@ -413,10 +444,18 @@ __device__ __forceinline__ void knn_coop_self(const TreeView<float, DIM>& tv, bo
         inleaf = box_min_dist2<float, DIM>(lb, q) <= cs.bd;
 #pragma unroll
         for (int d = 0; d < DIM; ++d) {
-          lqb.lo[d] = uniform_f(wave_min_f(inleaf ? q[d] : pos_inf<float>()));
-          lqb.hi[d] = uniform_f(wave_max_f(inleaf ? q[d] : -pos_inf<float>()));
+          const float lo = inleaf ? q[d] : pos_inf<float>();
+          const float hi = inleaf ? q[d] : -pos_inf<float>();
+          if constexpr (ROWBCAST) {
+            lqb.lo[d] = uniform_f(wave_min_f_bcast(lo));
+            lqb.hi[d] = uniform_f(wave_max_f_bcast(hi));
+          } else {
+            lqb.lo[d] = uniform_f(wave_min_f(lo));
+            lqb.hi[d] = uniform_f(wave_max_f(hi));
+          }
         }
-        rmax = uniform_f(wave_max_f(inleaf ? cs.bd : -1.0f));
+        if constexpr (ROWBCAST) rmax = uniform_f(wave_max_f_bcast(inleaf ? cs.bd : -1.0f));
+        else rmax = uniform_f(wave_max_f(inleaf ? cs.bd : -1.0f));
       }
     }
     cs.flush(part);

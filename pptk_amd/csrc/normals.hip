@@ -89,7 +89,7 @@ __global__ __launch_bounds__(kBlock) void normals_knn_kernel(TreeView<D, 3> tv,
   if (ok) knn_search_from_leaf<D, 3>(tv, pos, q, r2, -1, k, st);
   int n = 0;
 #pragma unroll
-  for (int s = 0; s < K; ++s) n += (s >= K - k && st.dist(s) < pos_inf<D>()) ? 1 : 0;
+  for (int s = 0; s < K; ++s) n += (s >= K - k && st.found(s)) ? 1 : 0;
   bool fb = false;
   if (n == 0) {
     write_nan<D, NE>(row, evecs, evals, sizes);
@@ -97,7 +97,7 @@ __global__ __launch_bounds__(kBlock) void normals_knn_kernel(TreeView<D, 3> tv,
     auto each = [&](auto f) {
 #pragma unroll
       for (int s = 0; s < K; ++s)
-        if (s >= K - k && st.dist(s) < pos_inf<D>()) f(tv.fwd[st.idx(s)]);
+        if (s >= K - k && st.found(s)) f(tv.fwd[st.idx(s)]);
     };
     double cd[3][3];
     float cf[3][3];
@@ -144,7 +144,7 @@ __global__ __launch_bounds__(kBlock, K <= 16 ? 8 : 1) void normals_coop_kernel(T
   int n = 0;
 #pragma unroll
   for (int s = 0; s < K; ++s) {
-    if (s >= K - k && st.dist(s) < pos_inf<float>()) {
+    if (s >= K - k && st.found(s)) {
       pos[n * 64] = tv.fwd[st.idx(s)];
       ++n;
     }

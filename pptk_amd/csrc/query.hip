@@ -52,7 +52,7 @@ __global__ __launch_bounds__(kBlock) void knn_kernel(TreeView<D, DIM> tv, int mo
   for (int s = 0; s < K; ++s) {
     if (s >= K - k) {
       const D ds = st.dist(s);
-      const bool f = ds < pos_inf<D>();
+      const bool f = st.found(s);
       orow[s - (K - k)] = f ? st.idx(s) : -1;
       if (out_dist) out_dist[row * k + (s - (K - k))] = f ? ds : pos_inf<D>();
       c += f ? 1 : 0;
@@ -86,7 +86,9 @@ __global__ __launch_bounds__(kBlock, K <= 16 && DIM <= 3 ? 8 : 1) void knn_coop_
   }
   TopK<float, K, false> st;
   st.init(k);
-  knn_coop_self<DIM, K>(tv, row >= 0, (int32_t)t, row, q, r2, k, true, st,
+  // at DIM 3 and K 16 this kernel holds 63 of its 64 vector registers; the
+  // row-broadcast reductions would spill there
+  knn_coop_self<DIM, K, !(DIM == 3 && K == 16)>(tv, row >= 0, (int32_t)t, row, q, r2, k, true, st,
                         lds + (threadIdx.x >> 6) * (coop_wave_bytes<K, DIM, 0>() / 8));
   if (row < 0) return;
   int32_t c = 0;
@@ -95,7 +97,7 @@ __global__ __launch_bounds__(kBlock, K <= 16 && DIM <= 3 ? 8 : 1) void knn_coop_
   for (int s = 0; s < K; ++s) {
     if (s >= K - k) {
       const float ds = st.dist(s);
-      const bool f = ds < pos_inf<float>();
+      const bool f = st.found(s);
       orow[s - (K - k)] = f ? st.idx(s) : -1;
       if (out_dist) out_dist[(int64_t)row * k + (s - (K - k))] = f ? ds : pos_inf<float>();
       c += f ? 1 : 0;

@@ -46,6 +46,8 @@ struct TopK {
     }
   }
   __device__ __forceinline__ D bound() const { return d[K - 1]; }
+  // slot s holds a neighbour
+  __device__ __forceinline__ bool found(int s) const { return d[s] < pos_inf<D>(); }
   __device__ __forceinline__ D dist(int s) const { return d[s]; }
   __device__ __forceinline__ int32_t idx(int s) const { return o[s]; }
   __device__ __forceinline__ bool beats(D dd, int32_t oo) const {
@@ -130,6 +132,11 @@ struct TopK<float, K, POS> {
   }
   __device__ __forceinline__ float bound() const {
     return __uint_as_float((uint32_t)(key[K - 1] >> 32));
+  }
+  // slot s holds a neighbour: a finite distance and an index other than -1
+  // (a search may start its slots at (bound, -1) instead of empty)
+  __device__ __forceinline__ bool found(int s) const {
+    return ((uint32_t)(key[s] >> 32) < 0x7f800000u) & ((uint32_t)key[s] != 0xffffffffu);
   }
   __device__ __forceinline__ bool beats(float dd, int32_t oo) const { return make(dd, oo) < key[K - 1]; }
 

@@ -258,6 +258,30 @@ def test_knn_float32_ties_and_denormal_distances(gpu):
         assert np.array_equal(got, bw)
 
 
+def test_knn_float32_self_dmax_short_rows(gpu):
+    """float32 self queries over every point with a finite dmax: a dense
+    cluster in a sparse background, so that many rows end before k
+    neighbours (padded with -1, distance +inf) and the search bound starts
+    from below dmax."""
+    rng = np.random.default_rng(11)
+    pts = np.concatenate([rng.random((30_000, 3)) * 0.1,
+                          rng.random((10_000, 3))]).astype(np.float32)
+    kd = _kd()
+    tree = kd._build(pts)
+    ot = O.OracleTree(pts)
+    for k in [4, 10, 16, 32]:
+        full = False
+        for dmax in [0.004, 0.02, 0.08]:
+            r = float(np.float32(dmax))
+            got, cnt = _rows(kd._query(tree, None, k=k, dmax=dmax), width=k)
+            want, wcnt = ot.knn(None, k, dmax=r)
+            assert np.array_equal(cnt, wcnt), (k, dmax)
+            assert (cnt < k).any(), (k, dmax)
+            assert np.array_equal(got, want), (k, dmax)
+            full = full or bool((cnt == k).any())
+        assert full, k
+
+
 @pytest.mark.parametrize("dtype", [np.int8, np.int32, np.uint16, np.int64, np.uint64])
 def test_knn_integer_dtypes(gpu, dtype):
     rng = np.random.default_rng(8)
@@ -557,6 +581,28 @@ def test_normals_vs_fp64_oracle(gpu, dtype, prec):
     ang = O.angle_deg(ev[:, 0], fev[:, 0])
     assert ang.max() < 0.5, ang.max()
     assert np.allclose(ew, few, rtol=1e-3, atol=1e-6 * np.abs(few).max())
+
+
+def test_normals_float32_k_with_radius_short_rows(gpu):
+    """float32 normals, k = 16 within a finite radius: many neighbourhoods end
+    before k points. Sizes match the oracle's exactly; the normals match an
+    FP64 solve of the oracle's neighbourhoods."""
+    import pptk_amd
+    rng = np.random.default_rng(12)
+    pts = np.concatenate([rng.random((15_000, 3)) * 0.1,
+                          rng.random((5_000, 3))]).astype(np.float32)
+    r = float(np.float32(0.02))
+    ev, ew, sz = pptk_amd.estimate_normals(pts, 16, 0.02, output_eigenvalues=True,
+                                           output_all_eigenvectors=True,
+                                           output_neighborhood_sizes=True, verbose=False)
+    ot = O.OracleTree(pts)
+    idx, cnt = ot.knn(pts, 16, dmax=r)   # the point itself included
+    assert np.array_equal(sz, cnt)
+    assert (cnt < 16).any() and (cnt == 16).any()
+    ok = cnt >= 4
+    fev, few = O.normals_fp64(pts, idx[ok], cnt[ok])
+    ang = O.angle_deg(ev[ok][:, 0], fev[:, 0])
+    assert np.median(ang) < 0.01 and np.percentile(ang, 90) < 0.5, np.percentile(ang, [50, 90])
 
 
 def test_normals_radius_and_k_minus_one(gpu):

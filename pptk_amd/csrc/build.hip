@@ -448,11 +448,13 @@ std::set<const void*> g_trees;
 // Measured on MI355X (10M-point lidar scene: build, kNN k = 16 of every
 // point, r-near of 1M points at 0.1 m, normals k = 16): the step takes
 // 16.0 / 15.4 / 14.8 / 14.6 / 14.5 / 14.3 ms at 64 / 80 / 96 / 112 / 128 /
-// 160 points per leaf and stays within noise up to 256. Point queries, which
-// scan leaves lane by lane, are 5% slower at 160 than at 96.
+// 160 points per leaf and stays within noise up to 256. With cheaper leaf
+// entries in the cooperative search it takes 14.45 / 14.27 / 14.12 / 14.22 ms
+// at 128 / 160 / 192 / 224. Point queries, which scan leaves lane by lane,
+// are 8% slower at 192 than at 96.
 int default_leaf_size(int dim, bool dbl, int64_t n) {
   (void)dim; (void)dbl; (void)n;
-  return 160;
+  return 192;
 }
 
 void register_tree(pptk_tree* t) {

@@ -11,6 +11,7 @@
 #include <memory>
 #include <string>
 #include <type_traits>
+#include <vector>
 
 #include "../../include/pptk_amd.h"
 
@@ -37,9 +38,38 @@ void clear_last_error();
 
 #define PK_LAUNCH_CHECK() PK_HIP(hipGetLastError())
 
+// One library call on this thread. Device blocks freed while it is active
+// go back to the pool behind a fence: the call's own stream may reuse them at
+// once, other streams only after an event recorded on the call's stream at
+// its end has completed. So an error that unwinds between a launch and its
+// synchronisation never hands an in-flight buffer to another stream. Scopes
+// nest; each fences its own blocks.
+class CallScope {
+ public:
+  CallScope();
+  ~CallScope();
+  CallScope(const CallScope&) = delete;
+  CallScope& operator=(const CallScope&) = delete;
+
+ private:
+  friend void scope_note_stream(hipStream_t s);
+  friend bool scope_defer_free(void* p);
+  friend hipStream_t scope_stream();
+  CallScope* outer_;
+  hipStream_t stream_ = nullptr;
+  bool has_stream_ = false;
+  bool many_streams_ = false;
+  std::vector<void*> unqueued_;   // freed before the call picked a stream
+  std::shared_ptr<void> fence_;   // the pool's fence for this call's blocks
+};
+
+// Records the stream the active call works on (no-op outside a call).
+void scope_note_stream(hipStream_t s);
+
 // Runs `body` and converts pk::Error / std::bad_alloc into a status code.
 template <typename F>
 pptk_status guard(F&& body) {
+  CallScope scope;
   try {
     clear_last_error();
     body();
@@ -62,12 +92,16 @@ pptk_status guard(F&& body) {
 void ensure_device();
 hipStream_t lib_stream();
 inline hipStream_t pick_stream(void* s) {
-  return s ? reinterpret_cast<hipStream_t>(s) : lib_stream();
+  if (!s) return lib_stream();
+  hipStream_t st = reinterpret_cast<hipStream_t>(s);
+  scope_note_stream(st);
+  return st;
 }
 int device_cu_count();
 
 // Pooled device allocation. Blocks are cached after free and reused for
-// requests of up to 1.25x smaller size; pptk_trim() returns them.
+// requests of up to 1.25x smaller size; pptk_trim() returns them. A block
+// freed inside a library call is fenced on that call's stream (CallScope).
 void* dev_alloc(size_t bytes);
 void dev_free(void* p);
 

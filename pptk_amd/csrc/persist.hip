@@ -19,9 +19,11 @@
 //
 // Loading validates every index on the host before anything reaches the
 // device (node ranges, child pairs, parents, leaf cover, the two row maps,
-// depth), so a damaged or hostile buffer is rejected with an error instead
-// of steering a kernel out of bounds.
+// depth and leaf count) and every node box (finite, holding its points and
+// its children's boxes), so a damaged or hostile buffer is rejected with an
+// error instead of steering a kernel out of bounds or pruning wrongly.
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <vector>
 
@@ -117,7 +119,53 @@ void validate(const FileHeader& h, const uint8_t* buf) {
     ++mapped;
   }
   if (mapped != nv) bad("forward map count");
-  (void)max_depth;
+  int64_t leaves = 0;
+  for (int64_t i = 0; i < nn; ++i) leaves += nodes[i].child == -1;
+  if (h.depth != max_depth + 1) bad("depth field");
+  if (h.nleaves != leaves) bad("leaf count field");
+}
+
+// Node boxes steer pruning, so a box that does not hold its node's points
+// would silently drop neighbours. Every non-empty node's box must be finite
+// with lo <= hi, a leaf's box must contain its points and an inner node's
+// box both child boxes. An empty node must carry the empty box
+// (lo = +inf, hi = -inf), which prunes it for every query.
+template <typename D>
+void validate_boxes(const FileHeader& h, const uint8_t* buf) {
+  const D* coords = (const D*)(buf + h.off_coords);
+  const D* boxes = (const D*)(buf + h.off_boxes);
+  const NodeRec* nodes = (const NodeRec*)(buf + h.off_nodes);
+  const int dim = h.dim;
+  const D inf = std::numeric_limits<D>::infinity();
+  auto bad = [](const char* what) { fail(PPTK_E_VALUE, "invalid tree buffer: %s", what); };
+  auto lo = [&](int64_t i, int d) { return boxes[i * 2 * dim + d]; };
+  auto hi = [&](int64_t i, int d) { return boxes[i * 2 * dim + dim + d]; };
+  for (int64_t i = 0; i < h.nn; ++i) {
+    const NodeRec& r = nodes[i];
+    if (r.a == r.b) {
+      for (int d = 0; d < dim; ++d)
+        if (!(lo(i, d) == inf && hi(i, d) == -inf)) bad("box of an empty node");
+      continue;
+    }
+    for (int d = 0; d < dim; ++d) {
+      const D l = lo(i, d), u = hi(i, d);
+      if (!(l - l == D(0)) || !(u - u == D(0)) || !(l <= u)) bad("node box");
+    }
+    if (r.child == -1) {
+      for (int32_t p = r.a; p < r.b; ++p)
+        for (int d = 0; d < dim; ++d) {
+          const D c = coords[(int64_t)p * dim + d];
+          if (c < lo(i, d) || c > hi(i, d)) bad("leaf box does not contain its points");
+        }
+    } else {
+      for (int64_t c = r.child; c <= (int64_t)r.child + 1; ++c) {
+        if (nodes[c].a == nodes[c].b) continue;
+        for (int d = 0; d < dim; ++d)
+          if (lo(c, d) < lo(i, d) || hi(c, d) > hi(i, d))
+            bad("node box does not contain a child box");
+      }
+    }
+  }
 }
 
 template <typename D, int DIM>
@@ -247,6 +295,8 @@ pptk_status api_tree_deserialize(const void* buf, int64_t size, void* stream, pp
     const bool finite = h.dist == PPTK_F64 ? all_finite((const double*)(b + h.off_coords), nc)
                                            : all_finite((const float*)(b + h.off_coords), nc);
     if (!finite) fail(PPTK_E_VALUE, "invalid tree buffer: coordinates");
+    if (h.dist == PPTK_F64) validate_boxes<double>(h, b);
+    else validate_boxes<float>(h, b);
     hipStream_t s = pick_stream(stream);
     std::unique_ptr<pptk_tree> t(new pptk_tree());
     t->dim = h.dim;

@@ -853,13 +853,14 @@ void rnear_fill_run(const pptk_tree* t, const pptk_queries* qs, double dmax, con
   oi.init(out_idx, (size_t)total * 4, out_mem);
   if (total > 0 && !out_idx) fail(PPTK_E_VALUE, "null output");
   const TreeView<D, DIM> tv = tree_view<D, DIM>(t);
+  // wave order of the fill; held until the stream has synchronised below
+  DevBuf wkeys, wvals;
   if (pq.m > 0 && total > 0) {
     const int pk = kRnearFillWidth;
     const int64_t nw = ceil_div(pq.m, pk);
     const unsigned grid = (unsigned)std::max<int64_t>(1, ceil_div(nw, kBlock / 64));
     // waves with the most output start first, so that the last waves to
     // start are short ones
-    DevBuf wkeys, wvals;
     const uint32_t* worder = nullptr;
     if (tv.nroots == 1 && nw > 1) {
       wkeys.reset((size_t)nw * 8);

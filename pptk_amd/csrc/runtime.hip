@@ -6,6 +6,7 @@
 #include <mutex>
 
 #include "common.hpp"
+#include "pool.hpp"
 
 namespace pk {
 
@@ -66,6 +67,7 @@ void ensure_device() {
 
 hipStream_t lib_stream() {
   ensure_device();
+  scope_note_stream(g_stream);
   return g_stream;
 }
 
@@ -77,59 +79,128 @@ int device_cu_count() {
 // ------------------------------------------------------------- allocator ----
 
 namespace {
-std::mutex g_pool_mu;
-std::multimap<size_t, void*> g_free;      // size -> block
-std::map<void*, size_t> g_live;           // block -> size
-size_t g_cached_bytes = 0;
 
-size_t round_size(size_t b) {
-  const size_t a = 256;
-  return (b + a - 1) / a * a;
+struct HipPoolDriver {
+  using Stream = hipStream_t;
+  using Event = hipEvent_t;
+  bool malloc(void** p, size_t n) {
+    if (hipMalloc(p, n) == hipSuccess) return true;
+    (void)hipGetLastError();
+    return false;
+  }
+  void free(void* p) { (void)hipFree(p); }
+  bool record(Event* ev, Stream s) {
+    if (hipEventCreateWithFlags(ev, hipEventDisableTiming) != hipSuccess) {
+      (void)hipGetLastError();
+      return false;
+    }
+    if (hipEventRecord(*ev, s) != hipSuccess) {
+      (void)hipGetLastError();
+      (void)hipEventDestroy(*ev);
+      return false;
+    }
+    return true;
+  }
+  bool done(Event ev) {
+    const hipError_t e = hipEventQuery(ev);
+    if (e == hipSuccess) return true;
+    (void)hipGetLastError();
+    return false;  // not ready yet, or an error: never reuse early
+  }
+  void wait(Event ev) {
+    if (hipEventSynchronize(ev) != hipSuccess) (void)hipGetLastError();
+  }
+  void destroy(Event ev) { (void)hipEventDestroy(ev); }
+  bool sync(Stream s) {
+    if (hipStreamSynchronize(s) == hipSuccess) return true;
+    (void)hipGetLastError();
+    return false;
+  }
+};
+
+HipPoolDriver g_drv;
+BlockPool<HipPoolDriver>& pool() {
+  static BlockPool<HipPoolDriver>* p = new BlockPool<HipPoolDriver>(g_drv);  // lives to exit
+  return *p;
 }
+
+thread_local CallScope* t_scope = nullptr;
+
 }  // namespace
+
+using Pool = BlockPool<HipPoolDriver>;
+
+CallScope::CallScope() : outer_(t_scope) { t_scope = this; }
+
+CallScope::~CallScope() {
+  t_scope = outer_;
+  // Blocks freed before any stream was picked were never used by this call.
+  for (void* p : unqueued_) pool().release_now(p);
+  if (!fence_) return;
+  auto f = std::static_pointer_cast<Pool::Fence>(fence_);
+  if (many_streams_) {
+    // Work of this call went to more than one stream: wait for all of it.
+    if (hipDeviceSynchronize() == hipSuccess) {
+      pool().pass_fence(f);
+      return;
+    }
+    (void)hipGetLastError();
+  }
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(stream_, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) {
+    // An event recorded now would join an unfinished capture: the fence stays
+    // open and its blocks stay with this stream.
+    (void)hipGetLastError();
+    return;
+  }
+  pool().close_fence(f, stream_);
+}
+
+void scope_note_stream(hipStream_t s) {
+  CallScope* c = t_scope;
+  if (!c) return;
+  if (c->has_stream_ && c->stream_ != s) c->many_streams_ = true;
+  if (!c->has_stream_) {
+    c->stream_ = s;
+    c->has_stream_ = true;
+  }
+}
+
+bool scope_defer_free(void* p) {
+  CallScope* c = t_scope;
+  if (!c) return false;
+  if (!c->has_stream_) {
+    c->unqueued_.push_back(p);
+    return true;
+  }
+  if (!c->fence_) c->fence_ = pool().open_fence();
+  pool().release_fenced(p, c->stream_, std::static_pointer_cast<Pool::Fence>(c->fence_));
+  return true;
+}
+
+hipStream_t scope_stream() {
+  CallScope* c = t_scope;
+  return c && c->has_stream_ ? c->stream_ : nullptr;
+}
 
 void* dev_alloc(size_t bytes) {
   ensure_device();
-  if (bytes == 0) bytes = 256;
-  size_t want = round_size(bytes);
-  {
-    std::lock_guard<std::mutex> lk(g_pool_mu);
-    auto it = g_free.lower_bound(want);
-    if (it != g_free.end() && it->first <= want + want / 4) {
-      void* p = it->second;
-      size_t sz = it->first;
-      g_free.erase(it);
-      g_cached_bytes -= sz;
-      g_live[p] = sz;
-      return p;
-    }
-  }
+  if (void* p = pool().take(bytes, scope_stream())) return p;
+  const size_t want = Pool::round_size(bytes);
   void* p = nullptr;
-  hipError_t e = hipMalloc(&p, want);
-  if (e != hipSuccess) {
+  if (!g_drv.malloc(&p, want)) {
     // Return cached blocks to the driver and try once more.
-    (void)hipGetLastError();
-    pptk_trim();
-    e = hipMalloc(&p, want);
-    if (e != hipSuccess) {
-      (void)hipGetLastError();
-      fail(PPTK_E_MEMORY, "device allocation of %zu bytes failed: %s", want,
-           hipGetErrorString(e));
-    }
+    pool().trim();
+    if (!g_drv.malloc(&p, want))
+      fail(PPTK_E_MEMORY, "device allocation of %zu bytes failed", want);
   }
-  std::lock_guard<std::mutex> lk(g_pool_mu);
-  g_live[p] = want;
+  pool().adopt(p, want);
   return p;
 }
 
 void dev_free(void* p) {
   if (!p) return;
-  std::lock_guard<std::mutex> lk(g_pool_mu);
-  auto it = g_live.find(p);
-  if (it == g_live.end()) return;
-  g_free.emplace(it->second, p);
-  g_cached_bytes += it->second;
-  g_live.erase(it);
+  if (!scope_defer_free(p)) pool().release_now(p);
 }
 
 // --------------------------------------------------------------- staging ----
@@ -191,13 +262,5 @@ extern "C" int pptk_device_count(void) {
 }
 
 extern "C" pptk_status pptk_trim(void) {
-  return pk::guard([] {
-    std::multimap<size_t, void*> blocks;
-    {
-      std::lock_guard<std::mutex> lk(pk::g_pool_mu);
-      blocks.swap(pk::g_free);
-      pk::g_cached_bytes = 0;
-    }
-    for (auto& kv : blocks) (void)hipFree(kv.second);
-  });
+  return pk::guard([] { pk::pool().trim(); });
 }

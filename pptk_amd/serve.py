@@ -400,28 +400,30 @@ class Server:
             result, error = None, None
             try:
                 if must_build:
+                    # Requests for the same cloud wait on entry.ready; it is
+                    # set only once entry.tree is assigned (or the build has
+                    # failed), never in between.
                     try:
                         tree = self.backend.build(req.points, slot)
                         nbytes = int(self.backend.tree_bytes(tree))
+                        with self._cv:
+                            entry.tree = tree
+                            entry.bytes = nbytes
+                            self._resident += nbytes
+                            # the tree's estimate and the build temporaries
+                            # are now the resident tree
+                            est = tree_bytes_estimate(entry.n) + build_temp_bytes(entry.n)
+                            self._reserved -= est
+                            need -= est
+                            self._cv.notify_all()
                     except BaseException as e:
                         entry.failed = e
+                        with self._cv:
+                            self._trees.pop(req.cloud, None)
+                            self._cv.notify_all()
                         raise
                     finally:
-                        if entry.failed is not None:
-                            with self._cv:
-                                self._trees.pop(req.cloud, None)
-                                self._cv.notify_all()
                         entry.ready.set()
-                    with self._cv:
-                        entry.tree = tree
-                        entry.bytes = nbytes
-                        self._resident += nbytes
-                        # the tree's estimate and the build temporaries are
-                        # now the resident tree
-                        est = tree_bytes_estimate(entry.n) + build_temp_bytes(entry.n)
-                        self._reserved -= est
-                        need -= est
-                        self._cv.notify_all()
                     t.built = True
                 else:
                     entry.ready.wait()
@@ -489,7 +491,12 @@ class GpuBackend:
         c = self.capi
         s = self._s(slot)
         if self.outputs == "device":
-            return self._run_device(req, tree, s, rnear_cap)
+            # Outputs are allocated on the slot stream, the stream the library
+            # writes them on, so torch's caching allocator orders their reuse
+            # against that work. A consumer on another stream calls
+            # tensor.record_stream() before freeing them.
+            with self.torch.cuda.stream(self.streams[slot]):
+                return self._run_device(req, tree, s, rnear_cap)
         if req.kind == "knn":
             return c.knn(tree, None, int(req.k), float(req.dmax), stream=s)
         if req.kind == "normals":
@@ -529,7 +536,6 @@ class GpuBackend:
             q = c.Queries(c.Q_INDICES, m, keep.data_ptr(), None, 0, c.MEM_DEVICE)
         off = torch.empty(m + 1, dtype=torch.int64, device=self.dev)
         total = ctypes.c_int64()
-        torch.cuda.current_stream().synchronize()  # the query copy above
         c.check(L.pptk_rnear_count(h, ctypes.byref(q), float(req.dmax),
                                    ctypes.c_void_p(off.data_ptr()), ctypes.byref(total),
                                    c.MEM_DEVICE, s))

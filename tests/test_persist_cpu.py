@@ -84,6 +84,11 @@ def test_intact_buffer_passes_validation():
         assert st == capi.E_DEVICE  # validated, then no device to upload to
 
 
+def _box(buf, off, node, side, d, val):
+    v = buf[off["off_boxes"]:].view(np.float32)
+    v[node * 6 + (3 if side == "hi" else 0) + d] = val
+
+
 def _i32(buf, off, idx, val):
     v = buf[off:].view(np.int32)
     v[idx] = val
@@ -92,6 +97,7 @@ def _i32(buf, off, idx, val):
 @pytest.mark.parametrize("damage", [
     "magic", "version", "dim", "counts", "short", "node_range", "child_even", "child_parent",
     "leafof", "orig", "fwd", "coords_nan", "cycle", "total",
+    "box_shrunk", "box_nan", "box_child", "box_inverted", "depth", "nleaves",
 ])
 def test_damaged_buffers_are_rejected(damage):
     from pptk_amd import capi
@@ -125,6 +131,18 @@ def test_damaged_buffers_are_rejected(damage):
     elif damage == "total":
         buf = np.concatenate([buf, np.zeros(8, np.uint8)])
         buf[256:264] = np.frombuffer(np.int64(buf.size + 64).tobytes(), np.uint8)
+    elif damage == "box_shrunk":
+        _box(buf, off, 4, "hi", 0, 5.5)                 # leaf 4 holds (6, 6, 6)
+    elif damage == "box_nan":
+        _box(buf, off, 1, "lo", 1, np.nan)
+    elif damage == "box_child":
+        _box(buf, off, 0, "lo", 2, 1.0)                 # root misses child 1 (z = 0)
+    elif damage == "box_inverted":
+        _box(buf, off, 3, "lo", 0, 6.0)                 # lo 6 > hi 5
+    elif damage == "depth":
+        buf[32:36] = np.frombuffer(np.int32(5).tobytes(), np.uint8)
+    elif damage == "nleaves":
+        buf[64:72] = np.frombuffer(np.int64(4).tobytes(), np.uint8)
     assert deserialize(buf) == capi.E_VALUE, damage
     assert b"invalid tree buffer" in capi.lib().pptk_last_error() or \
         b"unsupported" in capi.lib().pptk_last_error()

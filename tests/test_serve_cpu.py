@@ -316,3 +316,47 @@ def test_checkpoint_and_restore_resident_trees(tmp_path):
         assert same(r.result(timeout=30), brute(r.req, c))
     assert be2.builds == 0
     s2.close()
+
+
+class LockHoldingBackend(FakeBackend):
+    """Holds the server's lock from another thread for `hold` seconds, taken
+    just before a build returns. That widens the window between a finished
+    build and the assignment of its tree to the cloud entry, where requests
+    waiting for the same cloud must not yet run."""
+
+    def __init__(self, hold=0.1, **kw):
+        super().__init__(**kw)
+        self.hold = hold
+        self.none_trees = 0
+
+    def build(self, points, slot):
+        tree = super().build(points, slot)
+        got = threading.Event()
+
+        def hold():
+            with self.server._cv:
+                got.set()
+                time.sleep(self.hold)
+
+        threading.Thread(target=hold, daemon=True).start()
+        got.wait()
+        return tree
+
+    def run(self, req, tree, slot, rnear_cap):
+        if tree is None:
+            with self.lock:
+                self.none_trees += 1
+        return super().run(req, tree, slot, rnear_cap)
+
+
+def test_waiters_never_see_a_built_tree_before_it_is_published():
+    rng = np.random.default_rng(11)
+    c = rng.uniform(0, 1, (80, 3)).astype(np.float32)
+    be = LockHoldingBackend(hold=0.1, delay=0.05)
+    s = make_server(be, slots=4)
+    ts = [s.submit(serve.Request("knn", "c", c, k=4, tag=0))]
+    ts += [s.submit(serve.Request("knn", "c", None, k=4, tag=i)) for i in (1, 2, 3)]
+    for t in ts:
+        assert same(t.result(timeout=30), brute(t.req, c))
+    assert be.none_trees == 0 and be.builds == 1
+    s.close()

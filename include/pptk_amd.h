@@ -178,6 +178,22 @@ pptk_status pptk_knn(const pptk_tree* tree, const pptk_queries* q, int k, double
                      int32_t* out_idx, int32_t* out_cnt, void* out_dist,
                      pptk_mem out_mem, void* stream);
 
+/* Allocation-free kNN. No counterpart in pptk: for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Same results and arguments as pptk_knn
+ * with queries (PPTK_Q_INDICES / PPTK_Q_POINTS) and outputs in device memory;
+ * it calls no hipMalloc/hipFree. Its temporaries live in one 64 MiB
+ * device array per GPU, held from the library's load to the process's end:
+ * point and index query sets of 4096 or more queries are ordered in it, so
+ * they are limited to 2,739,072 queries per call (PPTK_E_VALUE beyond, with
+ * the limit in the message); k > 32 searches keep their heaps in it. Self
+ * queries of every point (PPTK_Q_ALL) need no scratch and have no limit.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (other threads, other streams) race and must be serialised by the
+ * caller. Returns after the results are complete on `stream`. */
+pptk_status pptk_knn_noalloc(const pptk_tree* tree, const pptk_queries* q, int k, double dmax,
+                             int32_t* out_idx, int32_t* out_cnt, void* out_dist, void* stream);
+
 /* r-near neighbours (d < dmax*dmax), two phases. Replaces
  * KdTree::RNearNeighbors / RNearNeighborsSelf (kdtree-impl.h:978-1069)
  * behind kdtree._query(k=None). Phase 1 writes CSR offsets (m+1 int64,

@@ -34,7 +34,8 @@ _NP2DT = {
 
 EXPORTED = [
     "pptk_last_error", "pptk_device_count", "pptk_trim", "pptk_tree_build", "pptk_tree_free",
-    "pptk_tree_check", "pptk_tree_get_info", "pptk_tree_export", "pptk_knn", "pptk_rnear_count",
+    "pptk_tree_check", "pptk_tree_get_info", "pptk_tree_export", "pptk_knn", "pptk_knn_noalloc",
+    "pptk_rnear_count",
     "pptk_rnear_fill", "pptk_normals", "pptk_sort_pairs_u64", "pptk_exclusive_scan_i32",
     "pptk_batch_create", "pptk_batch_run", "pptk_batch_get_info", "pptk_batch_free",
     "pptk_tree_serialize", "pptk_tree_deserialize",
@@ -103,6 +104,8 @@ def lib():
     L.pptk_tree_export.argtypes = [vp, vp, vp, vp, vp, vp, ci]
     L.pptk_knn.restype = ci
     L.pptk_knn.argtypes = [vp, ctypes.POINTER(Queries), ci, cd, vp, vp, vp, ci, vp]
+    L.pptk_knn_noalloc.restype = ci
+    L.pptk_knn_noalloc.argtypes = [vp, ctypes.POINTER(Queries), ci, cd, vp, vp, vp, vp]
     L.pptk_rnear_count.restype = ci
     L.pptk_rnear_count.argtypes = [vp, ctypes.POINTER(Queries), cd, vp, ctypes.POINTER(i64), ci,
                                    vp]

@@ -862,8 +862,7 @@ pptk_status api_batch_get_info(const pptk_batch* bp, pptk_batch_info* out) {
     out->device_bytes = (int64_t)(b->coff.bytes() + b->tiles.bytes() + b->tbeg.bytes() +
                                   b->tcnt.bytes() + b->tlo.bytes() + b->thi.bytes() +
                                   b->cinfo.bytes() + b->vcnt.bytes() + b->keys.bytes() +
-                                  b->vals.bytes() + b->ws.kalt.bytes() + b->ws.valt.bytes() +
-                                  b->ws.hist.bytes() + b->ws.offs.bytes() + b->ws.ssum.bytes() +
+                                  b->vals.bytes() + b->ws.bytes() +
                                   b->rec.bytes() + b->gorig.bytes() + b->fwd.bytes() + b->leafof.bytes() +
                                   b->nodes.bytes() + b->box.bytes() + b->lev_s.bytes() +
                                   b->lev_e.bytes() + b->done.bytes() + b->state.bytes());

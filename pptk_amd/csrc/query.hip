@@ -606,12 +606,32 @@ constexpr int64_t kOrderMin = 4096;
 // fewer radix passes than the full code.
 constexpr int kQueryOrderBits = 48;
 
+inline size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+// Scratch bytes that ordering m queries takes: the order itself, the sort
+// keys and the sort's temporaries.
+size_t order_scratch_bytes(int64_t m) {
+  return align256((size_t)m * 4) + align256((size_t)m * 8) + SortWorkspace::bytes_for(m);
+}
+
+// Largest query count whose order fits in `bytes` of scratch.
+int64_t max_ordered_queries(size_t bytes) {
+  int64_t lo = 0, hi = (int64_t)(bytes / 12) + 1;
+  while (hi - lo > 1) {
+    const int64_t mid = lo + (hi - lo) / 2;
+    if (order_scratch_bytes(mid) <= bytes) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
 template <typename D>
-void order_queries(const pptk_tree* t, PreparedQueries& pq, hipStream_t s) {
+void order_queries(const pptk_tree* t, PreparedQueries& pq, hipStream_t s,
+                   const QueryScratch* scratch) {
   const int64_t m = pq.m;
   if (m < kOrderMin || pq.mode == kQueryAll) return;
   OrderCache& oc = t->order_cache;
-  {
+  if (!scratch) {
     std::lock_guard<std::mutex> lk(oc.mu);
     if (oc.order && oc.src == pq.src && oc.m == m && oc.mode == pq.mode) {
       pq.order_buf = oc.order;
@@ -619,13 +639,35 @@ void order_queries(const pptk_tree* t, PreparedQueries& pq, hipStream_t s) {
       return;
     }
   }
-  DevBuf keys((size_t)m * 8);
-  auto buf = std::make_shared<DevBuf>((size_t)m * 4);
+  DevBuf keys_owned;
+  std::shared_ptr<DevBuf> buf;
+  uint32_t* ord = nullptr;
+  uint64_t* keys = nullptr;
+  SortWorkspace ws;
+  if (scratch) {
+    if (order_scratch_bytes(m) > scratch->bytes)
+      fail(PPTK_E_VALUE,
+           "%lld point or index queries need %zu bytes of scratch to be ordered; the "
+           "allocation-free entry point has %zu (at most %lld such queries per call)",
+           (long long)m, order_scratch_bytes(m), scratch->bytes,
+           (long long)max_ordered_queries(scratch->bytes));
+    uint8_t* p = (uint8_t*)scratch->base;
+    ord = (uint32_t*)p;
+    p += align256((size_t)m * 4);
+    keys = (uint64_t*)p;
+    p += align256((size_t)m * 8);
+    ws.bind(p, m);
+  } else {
+    keys_owned.reset((size_t)m * 8);
+    keys = keys_owned.as<uint64_t>();
+    buf = std::make_shared<DevBuf>((size_t)m * 4);
+    ord = buf->as<uint32_t>();
+    ws.reserve(m);
+  }
   int bits = 32;
   if (pq.mode == kQueryList) {
     query_position_kernel<<<grid_of(m), kBlock, 0, s>>>(
-        pq.dlist, t->fwd.as<const int32_t>(), m, t->n_input, keys.as<uint64_t>(),
-        buf->as<uint32_t>());
+        pq.dlist, t->fwd.as<const int32_t>(), m, t->n_input, keys, ord);
     PK_LAUNCH_CHECK();
   } else {
     BuildInfo<D> frame;
@@ -641,7 +683,7 @@ void order_queries(const pptk_tree* t, PreparedQueries& pq, hipStream_t s) {
       constexpr int full = DIM * MortonBits<DIM>::value;
       constexpr int shift = full > kQueryOrderBits ? full - kQueryOrderBits : 0;
       query_morton_kernel<D, DIM><<<grid_of(m), kBlock, 0, s>>>(
-          (const D*)pq.dpts, m, frame, shift, keys.as<uint64_t>(), buf->as<uint32_t>());
+          (const D*)pq.dpts, m, frame, shift, keys, ord);
       PK_LAUNCH_CHECK();
       bits = full - shift;
     };
@@ -649,9 +691,12 @@ void order_queries(const pptk_tree* t, PreparedQueries& pq, hipStream_t s) {
     else if (t->dim == 3) launch(std::integral_constant<int, 3>());
     else launch(std::integral_constant<int, 4>());
   }
-  radix_sort_pairs(keys.as<uint64_t>(), buf->as<uint32_t>(), m, bits, s);  // synchronises s
+  radix_sort_pairs_async(keys, ord, m, bits, ws, s);
+  pq.order = (const int32_t*)ord;
+  if (scratch) return;  // stream order keeps the scratch valid for the queries
+  // The cache entry is published only once the order is complete.
+  PK_HIP(hipStreamSynchronize(s));
   pq.order_buf = buf;
-  pq.order = buf->as<const int32_t>();
   std::lock_guard<std::mutex> lk(oc.mu);
   oc.src = pq.src;
   oc.m = m;
@@ -662,8 +707,11 @@ void order_queries(const pptk_tree* t, PreparedQueries& pq, hipStream_t s) {
 }  // namespace
 
 template <typename D>
-void prepare_queries(const pptk_tree* t, const pptk_queries* q, hipStream_t s, PreparedQueries& pq) {
+void prepare_queries(const pptk_tree* t, const pptk_queries* q, hipStream_t s, PreparedQueries& pq,
+                     const QueryScratch* scratch) {
   if (!q) fail(PPTK_E_VALUE, "null query set");
+  if (scratch && q->kind != PPTK_Q_ALL && q->mem != PPTK_MEM_DEVICE)
+    fail(PPTK_E_VALUE, "the allocation-free entry point takes queries in device memory");
   switch (q->kind) {
     case PPTK_Q_ALL:
       pq.mode = kQueryAll;
@@ -719,18 +767,21 @@ void prepare_queries(const pptk_tree* t, const pptk_queries* q, hipStream_t s, P
     default:
       fail(PPTK_E_VALUE, "unknown query kind %d", (int)q->kind);
   }
-  order_queries<D>(t, pq, s);
+  order_queries<D>(t, pq, s, scratch);
 }
-template void prepare_queries<float>(const pptk_tree*, const pptk_queries*, hipStream_t, PreparedQueries&);
-template void prepare_queries<double>(const pptk_tree*, const pptk_queries*, hipStream_t, PreparedQueries&);
+template void prepare_queries<float>(const pptk_tree*, const pptk_queries*, hipStream_t,
+                                     PreparedQueries&, const QueryScratch*);
+template void prepare_queries<double>(const pptk_tree*, const pptk_queries*, hipStream_t,
+                                      PreparedQueries&, const QueryScratch*);
 
 namespace {
 
 template <typename D, int DIM>
 void knn_run(const pptk_tree* t, const pptk_queries* qs, int k, double dmax, int32_t* out_idx,
-             int32_t* out_cnt, void* out_dist, pptk_mem out_mem, hipStream_t s) {
+             int32_t* out_cnt, void* out_dist, pptk_mem out_mem, hipStream_t s,
+             const QueryScratch* scratch = nullptr) {
   PreparedQueries pq;
-  prepare_queries<D>(t, qs, s, pq);
+  prepare_queries<D>(t, qs, s, pq, scratch);
   const D r = (D)dmax;
   const D r2 = r * r;
   const int64_t nrows = pq.nrows;
@@ -770,14 +821,28 @@ void knn_run(const pptk_tree* t, const pptk_queries* qs, int k, double dmax, int
       else if (k <= 16) launch(std::integral_constant<int, 16>());
       else launch(std::integral_constant<int, 32>());
     } else {
-      // Global-memory heaps; bound the distance workspace to ~256 MB.
-      const int64_t chunk = std::max<int64_t>(kBlock, ((int64_t)256 << 20) / ((int64_t)k * sizeof(D)));
-      DevBuf ws((size_t)std::min(chunk, m) * k * sizeof(D));
+      // Global-memory heaps; bound the distance workspace to ~256 MB, or to
+      // the scratch left after the query order.
+      int64_t chunk = std::max<int64_t>(kBlock, ((int64_t)256 << 20) / ((int64_t)k * sizeof(D)));
+      DevBuf ws;
+      D* wsp = nullptr;
+      if (scratch) {
+        const size_t used = pq.order ? align256((size_t)m * 4) : 0;
+        const int64_t fit = (int64_t)((scratch->bytes - used) / ((size_t)k * sizeof(D)));
+        chunk = fit / kBlock * kBlock;
+        if (chunk < kBlock)
+          fail(PPTK_E_VALUE, "k = %d needs more scratch than the allocation-free entry point has",
+               k);
+        wsp = (D*)((uint8_t*)scratch->base + used);
+      } else {
+        ws.reset((size_t)std::min(chunk, m) * k * sizeof(D));
+        wsp = ws.as<D>();
+      }
       for (int64_t t0 = 0; t0 < m; t0 += chunk) {
         const int64_t t1 = std::min(m, t0 + chunk);
         knn_large_kernel<D, DIM><<<grid_of(t1 - t0), kBlock, 0, s>>>(
             tv, pq.mode, pq.dlist, qp, pq.order, t0, t1, nrows, k, r2, (int32_t*)oi.dptr,
-            (int32_t*)oc.dptr, ws.as<D>(), (D*)od.dptr);
+            (int32_t*)oc.dptr, wsp, (D*)od.dptr);
         PK_LAUNCH_CHECK();
       }
       PK_HIP(hipStreamSynchronize(s));
@@ -920,6 +985,74 @@ extern "C" pptk_status pptk_knn(const pptk_tree* tree, const pptk_queries* q, in
                                 int32_t* out_idx, int32_t* out_cnt, void* out_dist,
                                 pptk_mem out_mem, void* stream) {
   return pk::api_knn(tree, q, k, dmax, out_idx, out_cnt, out_dist, out_mem, stream);
+}
+
+// ---------------------------------------------------- allocation-free kNN ----
+//
+// pptk_knn_noalloc runs the kNN kernels of pptk_knn with every temporary in
+// `scratch`, a file-scope device array each GPU holds from the library's
+// load to the process's end: the query order of point and index queries
+// (Morton keys, the order and the radix sort's temporaries, ~24.3 bytes per
+// query) and the heaps of k > 32 searches. Host code never passes `scratch`
+// itself to a kernel: its name is a host-side stand-in, so the device
+// address is looked up once per device with hipGetSymbolAddress and kept.
+// Every call on one GPU shares that GPU's array: calls that overlap on one
+// GPU race and must be serialised by the caller. No part of the array has
+// to start at zero (every kernel writes what it later reads).
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace pk {
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up on the first
+// call on that device.
+void* scratch_address() {
+  int dev = -1;
+  PK_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) fail(PPTK_E_DEVICE, "device id %d out of range", dev);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    PK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p || ((uintptr_t)p & 255u) != 0)
+      fail(PPTK_E_DEVICE, "scratch symbol address %p is not 256-byte aligned", p);
+    g_scratch_dev[dev] = p;
+  }
+  return g_scratch_dev[dev];
+}
+
+}  // namespace
+
+pptk_status api_knn_noalloc(const pptk_tree* tree, const pptk_queries* q, int k, double dmax,
+                            int32_t* out_idx, int32_t* out_cnt, void* out_dist, void* stream) {
+  return pk::guard([&] {
+    pk::check_tree(tree);
+    if (k < 1) pk::fail(PPTK_E_VALUE, "k must be positive (got %d)", k);
+    pk::check_dmax(dmax);
+    if (!q) pk::fail(PPTK_E_VALUE, "null query set");
+    pk::ensure_device();  // the tree's device is current
+    const QueryScratch sc{scratch_address(), kNoAllocScratchBytes};
+    hipStream_t s = pk::pick_stream(stream);
+    auto run = [&](auto dt, auto dimc) {
+      using D = typename decltype(dt)::type;
+      constexpr int DIM = decltype(dimc)::value;
+      pk::knn_run<D, DIM>(tree, q, k, dmax, out_idx, out_cnt, out_dist, PPTK_MEM_DEVICE, s, &sc);
+    };
+    pk::dispatch_tree(tree, run);
+  });
+}
+
+}  // namespace pk
+
+extern "C" pptk_status pptk_knn_noalloc(const pptk_tree* tree, const pptk_queries* q, int k,
+                                        double dmax, int32_t* out_idx, int32_t* out_cnt,
+                                        void* out_dist, void* stream) {
+  return pk::api_knn_noalloc(tree, q, k, dmax, out_idx, out_cnt, out_dist, stream);
 }
 
 namespace pk {

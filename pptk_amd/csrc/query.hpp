@@ -19,8 +19,19 @@ struct PreparedQueries {
   const int32_t* order = nullptr;
 };
 
+// Device memory a call may use for its temporaries instead of allocating
+// (the allocation-free entry point): the query order and its sort, and the
+// heaps of k > 32 searches.
+struct QueryScratch {
+  void* base = nullptr;  // 256-byte aligned
+  size_t bytes = 0;
+};
+
+// With `scratch`, the query order lives in it and the tree's order cache is
+// neither read nor written; queries must already be in device memory.
 template <typename D>
-void prepare_queries(const pptk_tree* t, const pptk_queries* q, hipStream_t s, PreparedQueries& pq);
+void prepare_queries(const pptk_tree* t, const pptk_queries* q, hipStream_t s, PreparedQueries& pq,
+                     const QueryScratch* scratch = nullptr);
 
 // Asynchronous device fill of n elements with v.
 template <typename T>

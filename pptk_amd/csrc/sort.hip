@@ -305,17 +305,34 @@ void exclusive_scan(const Tin* in, Tout* out, int64_t n, hipStream_t s) {
 template void exclusive_scan<int32_t, int64_t>(const int32_t*, int64_t*, int64_t, hipStream_t);
 template void exclusive_scan<uint32_t, uint32_t>(const uint32_t*, uint32_t*, int64_t, hipStream_t);
 
-void SortWorkspace::reserve(int64_t n) {
-  if (n <= n_cap) return;
+size_t SortWorkspace::bytes_for(int64_t n) {
   const int64_t tiles = ceil_div(std::max<int64_t>(n, 1), kTile);
   const int64_t hlen = (int64_t)kRadix * tiles;
   const int64_t scan_tiles = ceil_div(hlen, kScanTile);
-  kalt.reset((size_t)n * sizeof(uint64_t));
-  valt.reset((size_t)n * sizeof(uint32_t));
-  hist.reset((size_t)hlen * sizeof(uint32_t));
-  offs.reset((size_t)(hlen + 1) * sizeof(uint32_t));
-  ssum.reset((size_t)(scan_tiles + 1) * sizeof(uint32_t));
-  n_cap = n;
+  auto a = [](int64_t b) { return (size_t)((b + 255) / 256 * 256); };
+  return a(std::max<int64_t>(n, 1) * 8) + a(std::max<int64_t>(n, 1) * 4) + a(hlen * 4) +
+         a((hlen + 1) * 4) + a((scan_tiles + 1) * 4);
+}
+
+void SortWorkspace::bind(void* base, int64_t n) {
+  const int64_t m = std::max<int64_t>(n, 1);
+  const int64_t tiles = ceil_div(m, kTile);
+  const int64_t hlen = (int64_t)kRadix * tiles;
+  auto a = [](int64_t b) { return (size_t)((b + 255) / 256 * 256); };
+  uint8_t* p = (uint8_t*)base;
+  kalt = (uint64_t*)p; p += a(m * 8);
+  valt = (uint32_t*)p; p += a(m * 4);
+  hist = (uint32_t*)p; p += a(hlen * 4);
+  offs = (uint32_t*)p; p += a((hlen + 1) * 4);
+  ssum = (uint32_t*)p;
+  n_cap = m;
+}
+
+void SortWorkspace::reserve(int64_t n) {
+  if (n <= n_cap) return;
+  DevBuf buf(bytes_for(n));  // pool blocks are 256-byte aligned
+  bind(buf.get(), n);
+  owned_ = std::move(buf);
 }
 
 void radix_sort_pairs_async(uint64_t* keys, uint32_t* vals, int64_t n, int end_bit,
@@ -328,11 +345,11 @@ void radix_sort_pairs_async(uint64_t* keys, uint32_t* vals, int64_t n, int end_b
   const int64_t scan_tiles = ceil_div(hlen, kScanTile);
   uint64_t* k0 = keys;
   uint32_t* v0 = vals;
-  uint64_t* k1 = ws.kalt.as<uint64_t>();
-  uint32_t* v1 = ws.valt.as<uint32_t>();
-  uint32_t* hist = ws.hist.as<uint32_t>();
-  uint32_t* offs = ws.offs.as<uint32_t>();
-  uint32_t* ts = ws.ssum.as<uint32_t>();
+  uint64_t* k1 = ws.kalt;
+  uint32_t* v1 = ws.valt;
+  uint32_t* hist = ws.hist;
+  uint32_t* offs = ws.offs;
+  uint32_t* ts = ws.ssum;
   int passes = 0;
   for (int shift = 0; shift < end_bit; shift += 8, ++passes) {
     radix_hist_kernel<<<dim3((unsigned)tiles), dim3(kBlock), 0, s>>>(k0, n, shift, hist, (int)tiles);

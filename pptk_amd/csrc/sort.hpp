@@ -3,11 +3,27 @@
 
 namespace pk {
 
-// Preallocated temporaries of radix_sort_pairs_async for up to n_cap keys.
+// Temporaries of radix_sort_pairs_async for up to n_cap keys: either owned
+// pool buffers (reserve) or a slice of caller memory (bind).
 struct SortWorkspace {
-  DevBuf kalt, valt, hist, offs, ssum;
+  uint64_t* kalt = nullptr;
+  uint32_t* valt = nullptr;
+  uint32_t* hist = nullptr;
+  uint32_t* offs = nullptr;
+  uint32_t* ssum = nullptr;
   int64_t n_cap = 0;
+  // Allocates owned buffers for n keys (no-op if n <= n_cap).
   void reserve(int64_t n);
+  // Bytes of caller memory bind() needs for n keys.
+  static size_t bytes_for(int64_t n);
+  // Points the workspace at `base` (256-byte aligned, bytes_for(n) bytes),
+  // dropping any owned buffers.
+  void bind(void* base, int64_t n);
+  // Device bytes held (owned or bound).
+  size_t bytes() const { return n_cap > 0 ? bytes_for(n_cap) : 0; }
+
+ private:
+  DevBuf owned_;
 };
 
 // As radix_sort_pairs, with no allocation and no synchronisation (safe to

@@ -508,7 +508,7 @@ def main():
 
     import torch
 
-    from pptk_amd import capi, scenes
+    from pptk_amd import capi, pipeline, scenes
 
     n, k, dmax = args.n, args.k, float(np.float32(args.dmax))
     log("bench: generating lidar scene n=%d" % n)
@@ -519,60 +519,19 @@ def main():
     m = len(qpts)
 
     dev = torch.device("cuda:0")
-    stream = torch.cuda.Stream(device=dev)
-    sh = ctypes.c_void_p(stream.cuda_stream)
     pts_d = torch.from_numpy(pts).to(dev)
     q_d = torch.from_numpy(qpts).to(dev)
-    idx_d = torch.empty((n, k), dtype=torch.int32, device=dev)
-    cnt_d = torch.empty(n, dtype=torch.int32, device=dev)
-    nrm_d = torch.empty((n, 3), dtype=torch.float32, device=dev)
-    off_d = torch.empty(m + 1, dtype=torch.int64, device=dev)
-    ridx_d = None
-    torch.cuda.synchronize()
-    L = capi.lib()
-    qall, _ = capi.make_queries(None, mem=capi.MEM_DEVICE)
-    qall.kind = capi.Q_ALL
-    qpt, _ = capi.make_queries(dev_ptr=q_d.data_ptr(), m=m, kind=capi.Q_POINTS,
-                               dtype=np.float32)
     prec = capi.PREC_MIXED if args.precision == "mixed" else capi.PREC_FP64
-    phases = ["build", "knn", "rnear_count", "rnear_fill", "normals"]
-    state = {"rtotal": 0, "nfb": 0}
+    pipe = pipeline.LidarStep(pts_d, q_d, k=k, dmax=dmax, leaf=args.leaf, precision=prec)
+    phases = list(pipeline.LidarStep.PHASES)
 
     def step(record):
-        nonlocal ridx_d
-        ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(phases) + 1)]
-        with torch.cuda.stream(stream):
-            ev[0].record(stream)
-            tree = capi.Tree.build(dev_ptr=pts_d.data_ptr(), n=n, dim=3, dtype=np.float32,
-                                   leaf=args.leaf, stream=sh)
-            ev[1].record(stream)
-            capi.check(L.pptk_knn(ctypes.c_void_p(tree.handle), ctypes.byref(qall), k,
-                                  float("inf"), ctypes.c_void_p(idx_d.data_ptr()),
-                                  ctypes.c_void_p(cnt_d.data_ptr()), None, capi.MEM_DEVICE, sh))
-            ev[2].record(stream)
-            total = ctypes.c_int64()
-            capi.check(L.pptk_rnear_count(ctypes.c_void_p(tree.handle), ctypes.byref(qpt), dmax,
-                                          ctypes.c_void_p(off_d.data_ptr()), ctypes.byref(total),
-                                          capi.MEM_DEVICE, sh))
-            ev[3].record(stream)
-            if ridx_d is None or ridx_d.numel() < total.value:
-                ridx_d = torch.empty(max(1, int(total.value * 1.1)), dtype=torch.int32,
-                                     device=dev)
-            capi.check(L.pptk_rnear_fill(ctypes.c_void_p(tree.handle), ctypes.byref(qpt), dmax,
-                                         ctypes.c_void_p(off_d.data_ptr()),
-                                         ctypes.c_void_p(ridx_d.data_ptr()), capi.MEM_DEVICE,
-                                         sh))
-            ev[4].record(stream)
-            nfb = ctypes.c_int64()
-            capi.check(L.pptk_normals(ctypes.c_void_p(tree.handle), k, float("inf"), None, 0, 1,
-                                      prec, ctypes.c_void_p(nrm_d.data_ptr()), None, None,
-                                      ctypes.byref(nfb), capi.MEM_DEVICE, sh))
-            ev[5].record(stream)
-        tree.free()
-        state["rtotal"] = int(total.value)
-        state["nfb"] = int(nfb.value)
+        ev = None
         if record is not None:
-            stream.synchronize()
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(phases) + 1)]
+        pipe.run(ev)
+        if record is not None:
+            pipe.stream.synchronize()
             for i, p in enumerate(phases):
                 record.setdefault(p, []).append(ev[i].elapsed_time(ev[i + 1]))
 
@@ -591,6 +550,8 @@ def main():
     wall = time.perf_counter() - t0
     ms_step = wall * 1e3 / args.steps
     ph = {p: float(np.mean(v)) for p, v in rec.items()}
+    state = {"rtotal": pipe.rtotal, "nfb": pipe.num_fp64}
+    idx_d = pipe.idx_d
 
     ab = algorithmic_bytes(n, k, m, state["rtotal"])
     dom = max(PHASE_KERNEL, key=lambda p: ph[p])
@@ -614,8 +575,7 @@ def main():
                          "dmax=%g m on %d query points + normals k=%d r=inf (%s)"
                          % (n, k, args.dmax, m, k, args.precision)),
             "n_points": n, "k": k, "rnear_queries": m, "dmax": args.dmax,
-            "leaf_size": capi.Tree.build(pts[:1000]).info.leaf_size if args.leaf == 0
-            else args.leaf,
+            "leaf_size": pipe.leaf_size,
         },
         "phases_ms": ph,
         "knn_qps": n / (ph["knn"] * 1e-3),

@@ -47,8 +47,27 @@ def _load():
         lib.oracle_normals.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
                                        ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+        lib.oracle_csr_first_mismatch.restype = ctypes.c_longlong
+        lib.oracle_csr_first_mismatch.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_longlong,
+                                                                         ctypes.c_int]
         _LIB = lib
     return _LIB
+
+
+def csr_first_mismatch(off_a, idx_a, off_b, idx_b, nthreads=0):
+    """-1 if the two CSR lists hold the same set in every row, else the
+    first differing row (each row compared after sorting copies)."""
+    off_a = np.ascontiguousarray(off_a, np.int64)
+    off_b = np.ascontiguousarray(off_b, np.int64)
+    idx_a = np.ascontiguousarray(idx_a, np.int32)
+    idx_b = np.ascontiguousarray(idx_b, np.int32)
+    m = len(off_a) - 1
+    if len(off_b) - 1 != m:
+        raise ValueError("row counts differ")
+    if off_a[-1] > len(idx_a) or off_b[-1] > len(idx_b):
+        raise ValueError("offsets beyond the lists")
+    return int(_load().oracle_csr_first_mismatch(_ptr(off_a), _ptr(idx_a), _ptr(off_b),
+                                                 _ptr(idx_b), m, nthreads))
 
 
 def dist_dtype(dtype):

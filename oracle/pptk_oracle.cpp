@@ -640,4 +640,35 @@ void oracle_normals(void* hp, int k, double r, const int* rows, int m, double* e
   });
 }
 
+// Checker utility: whether two CSR neighbour lists hold the same set in every
+// row (rows compared after sorting copies; neither input is modified).
+// Returns -1 if every row matches, else the smallest differing row.
+long long oracle_csr_first_mismatch(const long long* off_a, const int* idx_a,
+                                    const long long* off_b, const int* idx_b, long long m,
+                                    int nthreads) {
+  long long first = -1;
+#pragma omp parallel num_threads(nthreads > 0 ? nthreads : omp_get_max_threads())
+  {
+    std::vector<int> a, b;
+    long long mine = -1;
+#pragma omp for schedule(dynamic, 1024)
+    for (long long i = 0; i < m; ++i) {
+      if (mine >= 0 && i > mine) continue;
+      const long long na = off_a[i + 1] - off_a[i], nb = off_b[i + 1] - off_b[i];
+      bool same = na == nb;
+      if (same && na > 0) {
+        a.assign(idx_a + off_a[i], idx_a + off_a[i + 1]);
+        b.assign(idx_b + off_b[i], idx_b + off_b[i + 1]);
+        std::sort(a.begin(), a.end());
+        std::sort(b.begin(), b.end());
+        same = a == b;
+      }
+      if (!same && (mine < 0 || i < mine)) mine = i;
+    }
+#pragma omp critical
+    if (mine >= 0 && (first < 0 || mine < first)) first = mine;
+  }
+  return first;
+}
+
 }  // extern "C"

@@ -1,0 +1,96 @@
+"""GPU: the 10M-point lidar workload at full size, through the path the
+benchmark times (pptk_amd.pipeline.LidarStep: tree built from a device
+pointer, every output left in device memory).
+
+* kNN k=16 self query: all 10,000,000 rows equal the oracle's rows, indices
+  and counts, ties broken by the smaller original index
+  (KNearestNeighborsSelf, pptk/kdtree/src/kdtree-impl.h:957-976).
+* r-near dmax=float32(0.1) on 1,000,000 query points (sampled without
+  replacement, seed 2, queried as points): every row holds the oracle's set
+  (RNearNeighbors, kdtree-impl.h:1008-1026; rows compared after sorting).
+* normals k=16, mixed precision: angle to FP64 normals of the same
+  neighbourhoods (oracle kNN including the point + LAPACK eigh) on 200,000
+  sampled rows; tolerance 0.5 degrees. The distribution is written to
+  $PPTK_REPORT_DIR/c1_full_parity.json when that directory exists.
+* Running the step again gives bit-identical outputs.
+"""
+import json
+import os
+import sys
+import time
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+REPORT_DIR = os.environ.get("PPTK_REPORT_DIR", "gpurun_out")
+N, M, K, DMAX = 10_000_000, 1_000_000, 16, 0.1
+
+
+def _say(*a):
+    print("c1_full:", *a, file=sys.stderr, flush=True)
+
+
+def _q(a):
+    return {"p50": float(np.percentile(a, 50)), "p90": float(np.percentile(a, 90)),
+            "p99": float(np.percentile(a, 99)), "max": float(np.max(a)), "n": int(a.size)}
+
+
+def test_c1_full_size_through_the_timed_path(gpu):
+    import torch
+
+    from pptk_amd import capi, pipeline, scenes
+    t0 = time.time()
+    pts = scenes.lidar_scene(N, seed=1)
+    qsel = np.random.default_rng(2).choice(N, size=M, replace=False)
+    qpts = np.ascontiguousarray(pts[qsel])
+    dev = torch.device("cuda:0")
+    step = pipeline.LidarStep(torch.from_numpy(pts).to(dev), torch.from_numpy(qpts).to(dev),
+                              k=K, dmax=DMAX, precision=capi.PREC_MIXED)
+    step.run()
+    idx, cnt, off, lst, nrm = step.host_outputs()
+    step.run()
+    again = step.host_outputs()
+    for a, b in zip((idx, cnt, off, lst, nrm), again):
+        assert np.array_equal(a, b, equal_nan=True), "second run differs"
+    del again
+    _say("GPU step done, %d r-near neighbours (%.1fs)" % (step.rtotal, time.time() - t0))
+
+    ot = O.OracleTree(pts)
+    _say("oracle built (%.1fs)" % (time.time() - t0))
+    widx, wcnt = ot.knn(None, K)
+    assert np.array_equal(cnt, wcnt)
+    bad = np.flatnonzero(np.any(idx != widx, axis=1))
+    assert bad.size == 0, "kNN rows differ, first %s" % bad[:5]
+    del widx, wcnt
+    _say("kNN: %d rows identical (%.1fs)" % (N, time.time() - t0))
+
+    woff, wlst = ot.rnear(qpts, float(np.float32(DMAX)))
+    assert np.array_equal(off, woff), "r-near counts differ"
+    first = O.csr_first_mismatch(off, lst, woff, wlst)
+    assert first == -1, "r-near row %d differs" % first
+    _say("r-near: %d rows, %d neighbours, identical sets (%.1fs)"
+         % (M, int(woff[-1]), time.time() - t0))
+    del wlst, lst
+
+    rows = np.random.default_rng(5).choice(N, size=200_000, replace=False)
+    nb, nc = ot.knn(pts[rows], K)  # the neighbourhood includes the point itself
+    fev, few = O.normals_fp64(pts, nb, nc)
+    ang = O.angle_deg(nrm[rows], fev[:, 0])
+    gap = (few[:, 1] - few[:, 0]) / few[:, 2]
+    report = {"workload": "lidar scene %d float32 points, normals k=%d mixed precision" % (N, K),
+              "sample_rows": int(rows.size), "angle_deg": _q(ang),
+              "fraction_eigengap_lt_1e-3": float(np.mean(gap < 1e-3)),
+              "angle_deg_eigengap_lt_1e-3": _q(ang[gap < 1e-3]) if np.any(gap < 1e-3) else None,
+              "fp64_fallback_fraction": step.num_fp64 / N,
+              "knn_rows_identical": N, "rnear_rows_identical": M,
+              "rnear_neighbours": int(woff[-1])}
+    if os.path.isdir(REPORT_DIR):
+        with open(os.path.join(REPORT_DIR, "c1_full_parity.json"), "w") as f:
+            json.dump(report, f, indent=1)
+    _say(json.dumps(report))
+    assert np.all(np.isfinite(ang))
+    assert report["angle_deg"]["max"] < 0.5

@@ -374,7 +374,10 @@ __global__ __launch_bounds__(kBlock) void roots_kernel(const uint32_t* __restric
 // with the point merged in at its (0, own index) rank, truncated to nk: the
 // same set, summed in the same order, as a separate search would give.
 template <typename D, int DIM, int K, int PREC, bool NORM>
-__global__ __launch_bounds__(kBlock) void batch_query_kernel(
+// Float forests with K <= 16 hold 7 waves per SIMD (<= 72 VGPRs): at 64
+// VGPRs the top-k merges spill, and without a bound the compiler's budget
+// drifts with the code around the search.
+__global__ __launch_bounds__(kBlock, (std::is_same<D, float>::value && K <= 16) ? 7 : 1) void batch_query_kernel(
                                                              TreeView<D, DIM> tv, int64_t N,
                                                              const uint32_t* __restrict__ vtotal,
                                                              const int32_t* __restrict__ gorig,
@@ -458,11 +461,9 @@ __global__ __launch_bounds__(kBlock) void batch_query_kernel(
         }
         if (pending && taken < nk) f((int32_t)t);
       };
-      double cd[3][3];
-      float cf[3][3];
       if constexpr (kCoop) {
         // staged in this lane's LDS buffer (free after the search), so the
-        // keys are dead before the covariance's two passes
+        // keys are dead before the covariance's passes
         int32_t* pl = coop_positions<DIM>(wl);
         int m = 0;
         merged([&](int32_t p) {
@@ -472,11 +473,10 @@ __global__ __launch_bounds__(kBlock) void batch_query_kernel(
         auto each = [&](auto f) {
           for (int j = 0; j < m; ++j) f(pl[j * 64]);
         };
-        knn_covariance<D, PREC>(tv, q, n, each, cd, cf);
+        fb = pca_row<D, PREC, 1>(tv, q, n, each, row, out_nrm, out_eval, nullptr);
       } else {
-        knn_covariance<D, PREC>(tv, q, n, merged, cd, cf);
+        fb = pca_row<D, PREC, 1>(tv, q, n, merged, row, out_nrm, out_eval, nullptr);
       }
-      fb = solve_and_write<D, PREC, 1>(row, n, cd, cf, out_nrm, out_eval, nullptr);
     }
     count_fallbacks(fb, nfb);
   }

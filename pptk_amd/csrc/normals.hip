@@ -99,10 +99,7 @@ __global__ __launch_bounds__(kBlock) void normals_knn_kernel(TreeView<D, 3> tv,
       for (int s = 0; s < K; ++s)
         if (s >= K - k && st.found(s)) f(tv.fwd[st.idx(s)]);
     };
-    double cd[3][3];
-    float cf[3][3];
-    knn_covariance<D, PREC>(tv, q, n, each, cd, cf);
-    fb = solve_and_write<D, PREC, NE>(row, n, cd, cf, evecs, evals, sizes);
+    fb = pca_row<D, PREC, NE>(tv, q, n, each, row, evecs, evals, sizes);
   }
   count_fallbacks(fb, nfb);
 }
@@ -156,10 +153,7 @@ __global__ __launch_bounds__(kBlock, K <= 16 ? 8 : 1) void normals_coop_kernel(T
     auto each = [&](auto f) {
       for (int j = 0; j < n; ++j) f(pos[j * 64]);
     };
-    double cd[3][3];
-    float cf[3][3];
-    knn_covariance<float, PREC>(tv, q, n, each, cd, cf);
-    fb = solve_and_write<float, PREC, NE>(row, n, cd, cf, evecs, evals, sizes);
+    fb = pca_row<float, PREC, NE>(tv, q, n, each, row, evecs, evals, sizes);
   }
   count_fallbacks(fb, nfb);
 }
@@ -196,10 +190,7 @@ __global__ __launch_bounds__(kBlock) void normals_heap_kernel(TreeView<D, 3> tv,
     auto each = [&](auto f) {
       for (int s = 0; s < n; ++s) f(hp[s]);
     };
-    double cd[3][3];
-    float cf[3][3];
-    knn_covariance<D, PREC>(tv, q, n, each, cd, cf);
-    fb = solve_and_write<D, PREC, NE>(row, n, cd, cf, evecs, evals, sizes);
+    fb = pca_row<D, PREC, NE>(tv, q, n, each, row, evecs, evals, sizes);
   }
   count_fallbacks(fb, nfb);
 }
@@ -223,18 +214,23 @@ __global__ __launch_bounds__(kBlock) void normals_rnear_kernel(TreeView<D, 3> tv
   bool ok;
   resolve_row<D>(tv, rows, t, q, row, pos, ok);
   if (row < 0) return;
-  int64_t n = 0;
-  double s1[3] = {0, 0, 0};
-  double s2[6] = {0, 0, 0, 0, 0, 0};
-  DF f1[3], f2[6];
+  // Shifted sums around q over the ball, in the precision P; returns n.
+  auto sums = [&](auto pc, double (&s1)[3], double (&s2)[6]) -> int64_t {
+    constexpr int P = decltype(pc)::value;
+    int64_t n = 0;
 #pragma unroll
-  for (int i = 0; i < 3; ++i) f1[i].zero();
+    for (int i = 0; i < 3; ++i) s1[i] = 0.0;
 #pragma unroll
-  for (int i = 0; i < 6; ++i) f2[i].zero();
-  if (ok) {
+    for (int i = 0; i < 6; ++i) s2[i] = 0.0;
+    DF f1[3], f2[6];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) f1[i].zero();
+#pragma unroll
+    for (int i = 0; i < 6; ++i) f2[i].zero();
+    if (!ok) return 0;
     auto visit = [&](const PointRec<D, 3>& r) {
       ++n;
-      if constexpr (PREC == PPTK_PREC_MIXED) {
+      if constexpr (P == PPTK_PREC_MIXED) {
         float e[3];
 #pragma unroll
         for (int a = 0; a < 3; ++a) { e[a] = (float)(r.c[a] - q[a]); f1[a].add(e[a]); }
@@ -249,20 +245,18 @@ __global__ __launch_bounds__(kBlock) void normals_rnear_kernel(TreeView<D, 3> tv
       }
     };
     rnear_visit<D, 3>(tv, root_of<D, 3>(tv, pos), q, r2, visit);
-  }
-  bool fb = false;
-  if (n == 0) {
-    write_nan<D, NE>(row, evecs, evals, sizes);
-  } else {
-    if constexpr (PREC == PPTK_PREC_MIXED) {
+    if constexpr (P == PPTK_PREC_MIXED) {
 #pragma unroll
       for (int i = 0; i < 3; ++i) s1[i] = f1[i].value();
 #pragma unroll
       for (int i = 0; i < 6; ++i) s2[i] = f2[i].value();
     }
+    return n;
+  };
+  auto covariance = [](int64_t n, const double (&s1)[3], const double (&s2)[6],
+                       double (&cd)[3][3]) {
     const double inv = 1.0 / (double)n;
     const double mu[3] = {s1[0] * inv, s1[1] * inv, s1[2] * inv};
-    double cd[3][3];
     cd[0][0] = s2[0] * inv - mu[0] * mu[0];
     cd[0][1] = s2[1] * inv - mu[0] * mu[1];
     cd[0][2] = s2[2] * inv - mu[0] * mu[2];
@@ -270,12 +264,38 @@ __global__ __launch_bounds__(kBlock) void normals_rnear_kernel(TreeView<D, 3> tv
     cd[1][2] = s2[4] * inv - mu[1] * mu[2];
     cd[2][2] = s2[5] * inv - mu[2] * mu[2];
     cd[1][0] = cd[0][1]; cd[2][0] = cd[0][2]; cd[2][1] = cd[1][2];
-    float cf[3][3];
+  };
+  double s1[3], s2[6], cd[3][3];
+  int64_t n = sums(std::integral_constant<int, PREC>{}, s1, s2);
+  bool fb = false;
+  if (n == 0) {
+    write_nan<D, NE>(row, evecs, evals, sizes);
+  } else {
+    covariance(n, s1, s2, cd);
+    bool solved = false;
+    if constexpr (PREC == PPTK_PREC_MIXED) {
+      float cf[3][3];
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+      for (int i = 0; i < 3; ++i)
 #pragma unroll
-      for (int j = 0; j < 3; ++j) cf[i][j] = (float)cd[i][j];
-    fb = solve_and_write<D, PREC, NE>(row, (int)n, cd, cf, evecs, evals, sizes);
+        for (int j = 0; j < 3; ++j) cf[i][j] = (float)cd[i][j];
+      float w[3], v[3][3];
+      eig_sym3<float>(cf, w, v);
+      if (!needs_fp64<NE>(w)) {
+        write_eig<D, float, NE>(row, w, v, (int)n, evecs, evals, sizes);
+        solved = true;
+      } else {
+        // below the gap threshold: the FP64 mode's sums over the same ball
+        n = sums(std::integral_constant<int, PPTK_PREC_FP64>{}, s1, s2);
+        covariance(n, s1, s2, cd);
+        fb = true;
+      }
+    }
+    if (!solved) {
+      double w[3], v[3][3];
+      eig_sym3<double>(cd, w, v);
+      write_eig<D, double, NE>(row, w, v, (int)n, evecs, evals, sizes);
+    }
   }
   count_fallbacks(fb, nfb);
 }

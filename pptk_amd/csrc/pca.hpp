@@ -143,22 +143,42 @@ __device__ __forceinline__ void knn_covariance(const TreeView<D, 3>& tv, const D
   }
 }
 
-// Solves and writes one row. Returns true if the FP64 fallback was taken.
-template <typename D, int PREC, int NE>
-__device__ __forceinline__ bool solve_and_write(int64_t row, int n, double (&cd)[3][3],
-                                                float (&cf)[3][3], D* evecs, D* evals,
-                                                int32_t* sizes) {
+// True if the FP32 eigenvalues (ascending) leave a gap below the threshold
+// that the written eigenvectors depend on: (l1 - l0) / l2 for the normal,
+// also (l2 - l1) / l2 when all three eigenvectors are written.
+template <int NE>
+__device__ __forceinline__ bool needs_fp64(const float (&w)[3]) {
+  const float l2 = w[2];
+  bool fb = !(l2 > 0.f) || ((double)(w[1] - w[0]) < kGapThreshold * (double)l2);
+  if (NE == 3) fb = fb || ((double)(w[2] - w[1]) < kGapThreshold * (double)l2);
+  return fb;
+}
+
+// PCA of one row whose neighbourhood `each` enumerates (tree positions, the
+// same order every time it is called). PPTK_PREC_FP64: FP64 covariance and
+// solve. PPTK_PREC_MIXED: FP32 covariance centred on q with compensated
+// sums and an FP32 solve; a row whose FP32 eigengap is below the threshold
+// recomputes its covariance in FP64 from the same neighbours and solves in
+// FP64. The FP32-centred covariance carries ~1e-7 relative rounding (each
+// centred offset is rounded to float), and a gap g amplifies a covariance
+// error e into an eigenvector error of about e / g, so the FP64 solve alone
+// would not give such rows FP64 accuracy; recomputed, they equal the FP64
+// mode's rows bit for bit. Returns true if the fallback was taken.
+template <typename D, int PREC, int NE, typename Each>
+__device__ __forceinline__ bool pca_row(const TreeView<D, 3>& tv, const D* q, int n, Each each,
+                                        int64_t row, D* evecs, D* evals, int32_t* sizes) {
+  double cd[3][3];
+  float cf[3][3];
   if constexpr (PREC == PPTK_PREC_MIXED) {
+    knn_covariance<D, PPTK_PREC_MIXED>(tv, q, n, each, cd, cf);
     float w[3], v[3][3];
     eig_sym3<float>(cf, w, v);
-    const float l2 = w[2];
-    bool fb = !(l2 > 0.f) || ((double)(w[1] - w[0]) < kGapThreshold * (double)l2);
-    if (NE == 3) fb = fb || ((double)(w[2] - w[1]) < kGapThreshold * (double)l2);
-    if (!fb) {
+    if (!needs_fp64<NE>(w)) {
       write_eig<D, float, NE>(row, w, v, n, evecs, evals, sizes);
       return false;
     }
   }
+  knn_covariance<D, PPTK_PREC_FP64>(tv, q, n, each, cd, cf);
   double w[3], v[3][3];
   eig_sym3<double>(cd, w, v);
   write_eig<D, double, NE>(row, w, v, n, evecs, evals, sizes);

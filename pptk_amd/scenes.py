@@ -11,8 +11,17 @@
   surfaces in proportion to their integrated density, plus isotropic
   N(0, 0.01 m) noise.
 * ``precision_stress``: the lidar scene in float64 translated by
-  (512000, 5400000, 300) m plus a near-isotropic cluster N(c, 0.05^2 I) of
-  2M points at c = (10, 10, 1) m before translation.
+  (512000, 5400000, 300) m plus a near-isotropic cluster of 2M points around
+  c = (10, 10, 1) m (before translation) with standard deviation 0.05 m per
+  axis. i.i.d. Gaussian points would give k=16 neighbourhoods with random
+  covariances, almost never with a small eigengap; the cluster is therefore
+  built from 125,000 groups of 16 points, each the isotropic 16-point design
+  (``isotropic_design16``) in a random orientation, 0.4 mm across, jittered
+  by a log-uniform 1e-9..1e-6 m per group, on the lattice points nearest c
+  (3.6 mm apart): every neighbourhood is one group and its eigengap
+  (l1 - l0) / l2 spreads from ~1e-6 to ~1e-2.
+* ``gap_clusters``: isolated float32 16-point clusters whose covariance has
+  eigenvalues (1, 1 + 4g, 4) in a random orientation, for chosen gaps g.
 * ``batched_clouds``: clouds of log-uniform size, each a random mix of noisy
   planes and spheres in a 10 m cube.
 """
@@ -120,13 +129,80 @@ def lidar_scene(n=10_000_000, seed=1, noise=0.01, dtype=np.float32):
     return pts.astype(dtype)
 
 
+def isotropic_design16():
+    """16 points with zero mean and second moment 0.625 I: the 8 cube
+    vertices (+-1, +-1, +-1) and the same at half size."""
+    v = np.array([[x, y, z] for x in (-1, 1) for y in (-1, 1) for z in (-1, 1)], np.float64)
+    return np.concatenate([v, 0.5 * v])
+
+
+def random_rotations(rng, m):
+    """m rotation matrices, uniform (QR of Gaussian matrices, signs fixed)."""
+    a = rng.normal(size=(m, 3, 3))
+    q, r = np.linalg.qr(a)
+    q = q * np.sign(np.diagonal(r, axis1=1, axis2=2))[:, None, :]
+    det = np.linalg.det(q)
+    q[det < 0, :, 0] *= -1
+    return q
+
+
+def _lattice_nearest(count, spacing):
+    """The `count` points of a cubic lattice of `spacing` nearest the origin."""
+    L = int(np.ceil((3 * count / (4 * np.pi)) ** (1 / 3))) + 2
+    g = np.arange(-L, L + 1)
+    pts = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3).astype(np.float64)
+    d = np.sum(pts * pts, axis=1)
+    order = np.lexsort((pts[:, 2], pts[:, 1], pts[:, 0], d))
+    return pts[order[:count]] * spacing
+
+
+def isotropic_groups(n_groups, centre, sigma=0.05, size=4e-4, seed=0):
+    """n_groups x 16 points: rotated isotropic designs of half-width `size`
+    on the lattice points nearest `centre`, spaced so that the cluster's
+    standard deviation per axis is `sigma`; each group jittered by
+    N(0, j^2) with j log-uniform in [1e-9, 1e-6] m."""
+    rng = np.random.default_rng(seed)
+    radius = sigma * np.sqrt(5.0)  # a uniform ball of this radius has std sigma
+    spacing = radius / (3 * n_groups / (4 * np.pi)) ** (1 / 3)
+    if spacing < 4 * size * np.sqrt(3.0):
+        raise ValueError("groups would overlap")
+    centres = np.asarray(centre, np.float64) + _lattice_nearest(n_groups, spacing)
+    rot = random_rotations(rng, n_groups)
+    design = isotropic_design16() * size
+    pts = centres[:, None, :] + np.einsum("gij,pj->gpi", rot, design)
+    jit = np.exp(rng.uniform(np.log(1e-9), np.log(1e-6), n_groups))
+    pts += rng.normal(size=pts.shape) * jit[:, None, None]
+    return pts.reshape(-1, 3)
+
+
 def precision_stress(n_scene=10_000_000, n_cluster=2_000_000, seed=1):
+    if n_cluster % 16:
+        raise ValueError("n_cluster must be a multiple of 16")
     base = lidar_scene(n_scene, seed=seed, dtype=np.float64)
-    rng = np.random.default_rng(seed + 100)
-    c = np.array([10.0, 10.0, 1.0])
-    cl = c + rng.normal(0.0, 0.05, (n_cluster, 3))
+    cl = isotropic_groups(n_cluster // 16, (10.0, 10.0, 1.0), seed=seed + 100)
     pts = np.concatenate([base, cl]) + np.array([512000.0, 5400000.0, 300.0])
     return pts
+
+
+def gap_clusters(gaps, per_gap=8, spacing=20.0, seed=0):
+    """float32 clusters of 16 points, isolated (`spacing` apart on a grid
+    centred at the origin): cluster j of gap g is the isotropic design mapped
+    to covariance R diag(1, 1 + 4g, 4) R^T, R a random rotation, so its
+    eigengap (l1 - l0) / l2 is g (before float32 rounding, which moves it by
+    ~1e-5). Returns (points (n, 3) float32, gap of each point's cluster)."""
+    rng = np.random.default_rng(seed)
+    gaps = np.repeat(np.asarray(gaps, np.float64), per_gap)
+    m = len(gaps)
+    side = int(np.ceil(m ** (1 / 3)))
+    g = (np.arange(side) - (side - 1) / 2) * spacing
+    grid = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)[:m]
+    rot = random_rotations(rng, m)
+    design = isotropic_design16() / np.sqrt(0.625)  # unit second moment
+    lam = np.stack([np.ones(m), 1 + 4 * gaps, np.full(m, 4.0)], 1)
+    local = design[None, :, :] * np.sqrt(lam)[:, None, :]
+    pts = grid[:, None, :] + np.einsum("gij,gpj->gpi", rot, local)
+    perm = rng.permutation(m * 16)
+    return (pts.reshape(-1, 3)[perm].astype(np.float32), np.repeat(gaps, 16)[perm])
 
 
 def batched_clouds(count=4096, seed=3, lo=1_000, hi=200_000, max_total=None):

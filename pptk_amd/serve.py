@@ -57,7 +57,7 @@ class Request:
     dmax: float = np.inf
     queries: np.ndarray = None
     tag: object = None
-    size: int = 0  # points of the cloud (for throughput accounting)
+    size: int = 0  # points the request queries (throughput accounting)
 
 
 class Cancelled(Exception):
@@ -578,13 +578,14 @@ def request_mix(rng, clouds, n_requests, follow_up=0.5):
             q = np.sort(rng.choice(n, size=min(n, 20_000), replace=False)).astype(np.int32)
         reqs.append(Request(kind, cid, pts, k=k if k else 16,
                             dmax=float(np.float32(dmax)) if np.isfinite(dmax) else dmax,
-                            queries=q, tag=i, size=n))
+                            queries=q, tag=i, size=n if q is None else len(q)))
     return reqs
 
 
 def percentile_stats(tickets, t0, t1):
-    """Requests/s, cloud points/s (each request counts its cloud's points)
-    and latency percentiles (submit to completion, queueing included)."""
+    """Requests/s, queried points/s (each request counts the points it
+    queries: its cloud for kNN and normals, its query set for r-near) and
+    latency percentiles (submit to completion, queueing included)."""
     lat = np.array([t.latency for t in tickets if t.error is None and t.latency is not None])
     pts = sum(t.req.size or (len(t.req.points) if t.req.points is not None else 0)
               for t in tickets if t.error is None)

@@ -1,12 +1,16 @@
 """GPU workload checks at full size.
 
 Precision stress: the 10M-point lidar scene in float64, translated by
-(512000, 5400000, 300) m, plus a 2M-point isotropic cluster (sigma 0.05 m).
-Normals k=16 in the mixed mode (FP32 centred double-float covariance, FP64
-eigensolve below the eigengap threshold) are compared with FP64 normals of
-the same neighbourhoods (oracle kNN + LAPACK) on seeded samples of scene and
-cluster rows. The angle-error distribution and the FP64-fallback fraction are
-written to $PPTK_REPORT_DIR/precision_report.json when that directory exists.
+(512000, 5400000, 300) m, plus a 2M-point near-isotropic cluster (sigma
+0.05 m) made of isotropic 16-point groups, whose neighbourhoods have
+eigengaps (l1 - l0) / l2 from ~1e-6 to ~1e-2 (pptk_amd.scenes). Normals k=16
+in the mixed mode (FP32 centred double-float covariance; rows below the 1e-3
+eigengap threshold recomputed and solved in FP64) are compared with FP64
+normals of the same neighbourhoods (oracle kNN + LAPACK) on seeded samples of
+scene and cluster rows, every sampled row within 0.5 degrees, the tiny-gap
+rows included. The angle-error distribution and the FP64-fallback fraction
+are written to $PPTK_REPORT_DIR/precision_report.json when that directory
+exists.
 """
 import json
 import os
@@ -65,10 +69,15 @@ def test_precision_stress_normals(gpu):
             "angle_deg_scene": _q(scene),
             "angle_deg_cluster": _q(clus),
             "angle_deg_eigengap_ge_1e-3": _q(well),
+            "angle_deg_eigengap_lt_1e-3": _q(ang[gap < 1e-3]),
         }
     report["fraction_eigengap_lt_1e-3"] = float(np.mean(gap < 1e-3))
+    report["sample_eigengap_quantiles"] = {
+        "p1": float(np.percentile(gap, 1)), "p10": float(np.percentile(gap, 10)),
+        "p50": float(np.percentile(gap, 50))}
     _report("precision_report.json", report)
     print(json.dumps(report, indent=1))
-    # target: within 0.5 degrees of FP64 wherever FP64 itself is well defined
-    assert report["mixed"]["angle_deg_eigengap_ge_1e-3"]["max"] < 0.5
-    assert report["fp64"]["angle_deg_eigengap_ge_1e-3"]["max"] < 0.5
+    assert report["fraction_eigengap_lt_1e-3"] > 0.1
+    # within 0.5 degrees of FP64 on every sampled row, tiny eigengaps included
+    assert report["mixed"]["angle_deg_all"]["max"] < 0.5
+    assert report["fp64"]["angle_deg_all"]["max"] < 0.5

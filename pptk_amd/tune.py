@@ -15,7 +15,7 @@ import os
 
 import numpy as np
 
-LEAVES = (8, 12, 16, 24, 32, 48, 64, 96, 128, 160, 192)
+LEAVES = (8, 12, 16, 24, 32, 48, 64, 96, 128, 160, 192, 224, 256)
 BLOCKS = (64, 128, 256)
 
 

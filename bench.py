@@ -744,7 +744,48 @@ def main():
                     help="batched: skip the plain-path comparison (profiling runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB mebibytes (counter calibration)")
+    ap.add_argument("--calibrate-scan", type=int, default=0, metavar="MIB",
+                    help="only run the library's exclusive scan over MIB mebibytes of int32 "
+                         "(reads its input twice, writes int64 offsets; counter calibration)")
+    ap.add_argument("--calibrate-gather", type=int, default=0, metavar="M",
+                    help="only gather M float32 elements from M distinct 128-byte lines of a "
+                         "1 GiB array (counter calibration of line granularity)")
     args = ap.parse_args()
+    if args.calibrate_scan:
+        import torch
+
+        from pptk_amd import capi
+        n = (args.calibrate_scan << 20) // 4
+        x = torch.ones(n, dtype=torch.int32, device="cuda:0")
+        out = torch.empty(n + 1, dtype=torch.int64, device="cuda:0")
+        torch.cuda.synchronize()
+        for _ in range(3):
+            capi.check(capi.lib().pptk_exclusive_scan_i32(
+                ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(out.data_ptr()), n,
+                capi.MEM_DEVICE))
+        torch.cuda.synchronize()
+        assert int(out[-1]) == n
+        print(json.dumps({"scan_elements": n, "bytes_read_per_scan": 8 * n,
+                          "bytes_written_per_scan": 8 * (n + 1), "scans": 3}), flush=True)
+        return
+    if args.calibrate_gather:
+        import torch
+        m = args.calibrate_gather
+        lines = (1 << 30) // 128
+        if m > lines:
+            raise SystemExit("at most %d lines" % lines)
+        src = torch.ones((1 << 30) // 4, dtype=torch.float32, device="cuda:0")
+        g = torch.Generator(device="cpu").manual_seed(9)
+        line = torch.randperm(lines, generator=g)[:m]
+        idx = (line * 32 + torch.randint(0, 32, (m,), generator=g)).to("cuda:0")
+        torch.cuda.synchronize()
+        for _ in range(3):
+            out = torch.index_select(src, 0, idx)
+        torch.cuda.synchronize()
+        print(json.dumps({"gathered": m, "distinct_lines_128B": m, "index_bytes": 8 * m,
+                          "out_bytes": 4 * m, "gathers": 3, "sum": float(out.sum())}),
+              flush=True)
+        return
     if args.calibrate_copy:
         import torch
         nbytes = args.calibrate_copy << 20

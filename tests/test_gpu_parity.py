@@ -143,6 +143,40 @@ def test_knn_self_c0_float64_bit_exact(gpu):
     assert np.array_equal(got, want)
 
 
+def test_normals_c0_float64_every_row_vs_fp64(gpu):
+    """The uniform-cube workload's normals leg on its own input: 100k float64
+    points (seed 0), estimate_normals(pts, 16, inf) (FP64 for float64 input,
+    as the reference computes in T = double, estimate_normals.cpp:84-89),
+    against LAPACK on the oracle's neighbourhoods, every row. The angle
+    distribution goes to $PPTK_REPORT_DIR/c0_normals.json when that
+    directory exists."""
+    import json
+    import os
+
+    import pptk_amd
+    pts = np.random.RandomState(0).rand(100_000, 3)
+    ev, ew, sz = pptk_amd.estimate_normals(pts, 16, np.inf, output_eigenvalues=True,
+                                           output_neighborhood_sizes=True, verbose=False)
+    assert ev.dtype == np.float64 and ev.shape == (100_000, 3)
+    nb, cnt = O.OracleTree(pts).knn(pts, 16)
+    assert np.array_equal(sz, cnt)
+    fev, few = O.normals_fp64(pts, nb, cnt)
+    ang = O.angle_deg(ev, fev[:, 0])
+    q = {p: float(np.percentile(ang, v)) for p, v in (("p50", 50), ("p90", 90), ("p99", 99))}
+    q["max"] = float(ang.max())
+    gap = (few[:, 1] - few[:, 0]) / few[:, 2]
+    report = {"workload": "uniform cube 100000 float64 points (seed 0), normals k=16 r=inf",
+              "rows": 100_000, "angle_deg": q,
+              "fraction_eigengap_lt_1e-3": float(np.mean(gap < 1e-3))}
+    d = os.environ.get("PPTK_REPORT_DIR", "gpurun_out")
+    if os.path.isdir(d):
+        with open(os.path.join(d, "c0_normals.json"), "w") as f:
+            json.dump(report, f, indent=1)
+    assert q["max"] < 0.5, q
+    # smallest eigenvalue to FP64 accuracy relative to the largest
+    assert np.all(np.abs(ew - few[:, 0]) <= 1e-9 * few[:, 0] + 1e-12 * few[:, 2])
+
+
 @pytest.mark.parametrize("k", [1, 2, 5, 8, 16, 17, 32, 33, 64])
 def test_knn_self_float32_k(gpu, k):
     rng = np.random.default_rng(k)

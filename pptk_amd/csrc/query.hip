@@ -516,16 +516,14 @@ __global__ __launch_bounds__(kBlock) void rnear_packet_kernel(TreeView<D, DIM> t
   }
 }
 
-// Sort key of each r-near fill wave (pk consecutive queries): the
-// complement of its output count, so that an ascending sort puts the waves
-// with the most output first. Thread t serves row order[t] (or t); for
+// Cost key of each r-near fill wave (pk consecutive queries): the
+// complement of its output count. Thread t serves row order[t] (or t); for
 // self queries of every point (row_of_pos set) it serves row_of_pos[t].
 __global__ __launch_bounds__(kBlock) void rnear_wave_cost_kernel(const int32_t* __restrict__ order,
                                                                  const int32_t* __restrict__ row_of_pos,
                                                                  const int64_t* __restrict__ offsets,
                                                                  int64_t m, int64_t nrows, int pk,
-                                                                 uint64_t* __restrict__ keys,
-                                                                 uint32_t* __restrict__ vals) {
+                                                                 uint64_t* __restrict__ keys) {
   const int64_t w = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (w * pk >= m) return;
   int64_t c = 0;
@@ -539,7 +537,68 @@ __global__ __launch_bounds__(kBlock) void rnear_wave_cost_kernel(const int32_t* 
     }
   }
   keys[w] = 0xffffffffull - (uint64_t)(c < 0xffffffffll ? c : 0xffffffffll);
-  vals[w] = (uint32_t)w;
+}
+
+// Launch order of the fill's waves, heaviest first: waves bucketed by
+// floor(log2(output + 1)) from their rnear_wave_cost_kernel keys, buckets in
+// decreasing order, waves in index order inside a bucket. One block; each
+// wave of the block ranks a contiguous span of keys with ballots (no atomics).
+// One launch in place of a 32-bit radix sort of the keys (4 passes, 20
+// launches).
+constexpr int kBucketBlock = 1024;
+constexpr int kBuckets = 33;
+__global__ __launch_bounds__(kBucketBlock) void rnear_wave_bucket_kernel(
+    const uint64_t* __restrict__ keys, int64_t nw, uint32_t* __restrict__ worder) {
+  constexpr int kWaves = kBucketBlock / 64;
+  __shared__ uint32_t cnt[kWaves][kBuckets];
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const int64_t span = (nw + kWaves - 1) / kWaves;
+  const int64_t lo = wv * span;
+  const int64_t hi = lo + span < nw ? lo + span : nw;
+  auto bucket = [](uint64_t key) {
+    const uint64_t c = 0xffffffffull - key;  // the wave's output, < 2^32
+    return 63 - __clzll(c + 1ull);           // 0 .. 32
+  };
+  // the buckets present among 64 keys, one ballot each
+  auto each_bucket = [&](int b, auto f) {
+    uint64_t todo = __ballot(b >= 0);
+    while (todo) {
+      const int k = __builtin_amdgcn_readlane(b, __builtin_ctzll(todo));
+      const uint64_t m = __ballot(b == k);
+      todo &= ~m;
+      f(k, m);
+    }
+  };
+  uint32_t mine = 0;  // lane b counts bucket b of this span (kBuckets <= 64)
+  for (int64_t i0 = lo; i0 < hi; i0 += 64) {
+    const int64_t i = i0 + lane;
+    const int b = i < hi ? bucket(keys[i]) : -1;
+    each_bucket(b, [&](int k, uint64_t m) { mine += lane == k ? (uint32_t)__popcll(m) : 0u; });
+  }
+  if (lane < kBuckets) cnt[wv][lane] = mine;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    // exclusive offsets: buckets from the heaviest down, spans in order
+    uint32_t run = 0;
+    for (int k = kBuckets - 1; k >= 0; --k)
+      for (int v = 0; v < kWaves; ++v) {
+        const uint32_t c = cnt[v][k];
+        cnt[v][k] = run;
+        run += c;
+      }
+  }
+  __syncthreads();
+  uint32_t base = lane < kBuckets ? cnt[wv][lane] : 0u;  // lane b: next slot of bucket b
+  for (int64_t i0 = lo; i0 < hi; i0 += 64) {
+    const int64_t i = i0 + lane;
+    const int b = i < hi ? bucket(keys[i]) : -1;
+    each_bucket(b, [&](int k, uint64_t m) {
+      const uint32_t bk = (uint32_t)__builtin_amdgcn_readlane((int)base, k);
+      if (b == k) worder[bk + rank_below(m)] = (uint32_t)i;
+      base += lane == k ? (uint32_t)__popcll(m) : 0u;
+    });
+  }
 }
 
 template <typename T>
@@ -932,9 +991,11 @@ void rnear_fill_run(const pptk_tree* t, const pptk_queries* qs, double dmax, con
       wvals.reset((size_t)nw * 4);
       rnear_wave_cost_kernel<<<grid_of(nw), kBlock, 0, s>>>(
           pq.order, pq.mode == kQueryAll ? tv.orig : nullptr, (const int64_t*)doff.ptr, pq.m,
-          nrows, pk, wkeys.as<uint64_t>(), wvals.as<uint32_t>());
+          nrows, pk, wkeys.as<uint64_t>());
       PK_LAUNCH_CHECK();
-      radix_sort_pairs(wkeys.as<uint64_t>(), wvals.as<uint32_t>(), nw, 32, s);
+      rnear_wave_bucket_kernel<<<1, kBucketBlock, 0, s>>>(wkeys.as<uint64_t>(), nw,
+                                                           wvals.as<uint32_t>());
+      PK_LAUNCH_CHECK();
       worder = wvals.as<uint32_t>();
     }
     if (tv.nroots == 1 && pq.mode == kQueryPoints)

@@ -1,0 +1,125 @@
+"""GPU: r-near count and fill passes around their edges, against the oracle
+(RNearNeighbors, pptk/kdtree/src/kdtree-impl.h:978-1069; lists compared as
+sets):
+* the query array rewritten in place between count and fill (same pointer,
+  same count): the fill reuses the count's processing order (a permutation
+  kept for locality) and must still answer the rewritten rows;
+* balls that straddle every leaf of a thin shell (more than 10M neighbours,
+  most from partly covered leaves) next to balls that cover a whole blob;
+* a count on other queries, or at another dmax, between a count and its
+  fill; self queries of every point through the drop-in.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _count(tree, q, dmax):
+    from pptk_amd import capi
+    off = np.empty(q.m + 1, np.int64)
+    tot = ctypes.c_int64()
+    capi.check(capi.lib().pptk_rnear_count(ctypes.c_void_p(tree.handle), ctypes.byref(q),
+                                           float(dmax), capi._p(off), ctypes.byref(tot),
+                                           capi.MEM_HOST, None))
+    return off
+
+
+def _fill(tree, q, dmax, off):
+    from pptk_amd import capi
+    idx = np.empty(max(int(off[-1]), 1), np.int32)
+    capi.check(capi.lib().pptk_rnear_fill(ctypes.c_void_p(tree.handle), ctypes.byref(q),
+                                          float(dmax), capi._p(off), capi._p(idx),
+                                          capi.MEM_HOST, None))
+    return idx[: int(off[-1])]
+
+
+def _same_sets(off, idx, woff, widx):
+    assert np.array_equal(off, woff)
+    assert O.csr_first_mismatch(off, idx, woff, widx) == -1
+
+
+def test_fill_after_rewriting_the_queries_in_place(gpu):
+    """Rows i and j of equal count swapped in place after the count: the old
+    offsets still fit, and the fill must answer the rewritten rows."""
+    from pptk_amd import capi
+    rng = np.random.default_rng(41)
+    pts = rng.random((200_000, 3)).astype(np.float32)
+    tree = capi.Tree.build(pts)
+    qp = np.ascontiguousarray(pts[rng.choice(len(pts), 20_000, replace=False)])
+    q, keep = capi.make_queries(qp)
+    dmax = float(np.float32(0.02))
+    off = _count(tree, q, dmax)
+    cnt = np.diff(off)
+    # swap pairs of rows with equal counts but different neighbours
+    swapped = 0
+    for c in np.unique(cnt[cnt > 3])[:50]:
+        rows = np.flatnonzero(cnt == c)
+        if len(rows) >= 2:
+            i, j = rows[0], rows[1]
+            keep[[i, j]] = keep[[j, i]]
+            swapped += 1
+    assert swapped >= 10
+    idx = _fill(tree, q, dmax, off)
+    ot = O.OracleTree(pts)
+    woff, widx = ot.rnear(keep, dmax)
+    _same_sets(off, idx, woff, widx)
+    tree.free()
+
+
+@pytest.mark.parametrize("m_centre", [16, 48])
+def test_straddling_shell_and_covered_blob(gpu, m_centre):
+    """Queries at the centre of a thin spherical shell with the shell's
+    radius: every leaf of the shell straddles every such ball. Queries inside
+    a separate blob take it whole."""
+    from pptk_amd import capi
+    rng = np.random.default_rng(42)
+    d = rng.normal(size=(100_000, 3))
+    shell = d / np.linalg.norm(d, axis=1, keepdims=True) * (1.0 + rng.normal(0, 1e-3, (100_000, 1)))
+    blob = 5.0 + 0.5 * rng.random((50_000, 3))
+    pts = np.concatenate([shell, blob]).astype(np.float32)
+    tree = capi.Tree.build(pts)
+    qp = np.concatenate([rng.normal(0, 1e-3, (m_centre, 3)),
+                         blob[rng.choice(len(blob), 200, replace=False)]]).astype(np.float32)
+    q, keep = capi.make_queries(qp)
+    dmax = float(np.float32(1.0))
+    off = _count(tree, q, dmax)
+    idx = _fill(tree, q, dmax, off)
+    woff, widx = O.OracleTree(pts).rnear(keep, dmax)
+    assert woff[-1] > 10_000_000
+    _same_sets(off, idx, woff, widx)
+    tree.free()
+
+
+def test_interleaved_counts_and_another_dmax(gpu):
+    from pptk_amd import capi
+    rng = np.random.default_rng(43)
+    pts = rng.random((100_000, 3)).astype(np.float32)
+    tree = capi.Tree.build(pts)
+    ot = O.OracleTree(pts)
+    qa = np.ascontiguousarray(pts[:8000])
+    qb = np.ascontiguousarray(pts[50_000:58_000])
+    a, ka = capi.make_queries(qa)
+    b, kb = capi.make_queries(qb)
+    d1, d2 = float(np.float32(0.03)), float(np.float32(0.05))
+    offa = _count(tree, a, d1)
+    offb = _count(tree, b, d1)
+    _same_sets(offa, _fill(tree, a, d1, offa), *ot.rnear(qa, d1))
+    _same_sets(offb, _fill(tree, b, d1, offb), *ot.rnear(qb, d1))
+    # a count at one dmax, a fill at another (with that dmax's offsets)
+    _count(tree, a, d1)
+    w2 = ot.rnear(qa, d2)
+    _same_sets(w2[0], _fill(tree, a, d2, w2[0]), *w2)
+    # self queries of every point: count then fill through the drop-in
+    import pptk_amd
+    kd = pptk_amd.kdtree
+    t2 = kd._build(pts)
+    lists = kd._query(t2, None, k=None, dmax=d1)
+    off = np.concatenate([[0], np.cumsum([len(x) for x in lists])]).astype(np.int64)
+    woff, widx = ot.rnear(None, d1)
+    _same_sets(off, np.concatenate(lists), woff, widx)
+    tree.free()

@@ -342,25 +342,20 @@ void build_impl(pptk_tree* t, const T* dpts, int64_t n, hipStream_t s) {
   // range. Depth is bounded by the key bits plus 31 halvings of equal keys.
   constexpr int kLevelBatch = 8;
   const int max_levels = DIM * MortonBits<DIM>::value + 33;
-  DevBuf lev((size_t)(2 * (max_levels + 1)) * 4);
+  // level table [lev_s | lev_e] followed by the LevelState, read back in one
+  // copy at the end
+  const size_t lev_bytes = (size_t)(2 * (max_levels + 1)) * 4;
+  const size_t st_off = (lev_bytes + 15) / 16 * 16;
+  DevBuf lev(st_off + sizeof(LevelState));
   DevBuf done((size_t)max_levels * 4);
-  DevBuf state(sizeof(LevelState));
   int32_t* lev_s = lev.as<int32_t>();
   int32_t* lev_e = lev_s + (max_levels + 1);
-  {
-    NodeRec root;
-    root.a = 0; root.b = (int32_t)nv; root.child = -1; root.parent = -1;
-    LevelState st0;
-    std::memset(&st0, 0, sizeof(st0));
-    st0.counter = 1;
-    const int32_t l0[2] = {0, 1};
-    PK_HIP(hipMemcpyAsync(nodes.get(), &root, sizeof(root), hipMemcpyHostToDevice, s));
-    PK_HIP(hipMemcpyAsync(state.get(), &st0, sizeof(st0), hipMemcpyHostToDevice, s));
-    PK_HIP(hipMemsetAsync(lev.get(), 0, lev.bytes(), s));
-    PK_HIP(hipMemsetAsync(done.get(), 0, done.bytes(), s));
-    PK_HIP(hipMemcpyAsync(lev_s, &l0[0], 4, hipMemcpyHostToDevice, s));
-    PK_HIP(hipMemcpyAsync(lev_e, &l0[1], 4, hipMemcpyHostToDevice, s));
-  }
+  LevelState* state_d = (LevelState*)((char*)lev.get() + st_off);
+  // root node, state and level table set by one kernel: no host copies
+  level_init_kernel<<<1, kLevelBlock, 0, s>>>(nodes.as<NodeRec>(), state_d, lev_s,
+                                               lev_e, done.as<uint32_t>(), max_levels,
+                                               (int32_t)nv);
+  PK_LAUNCH_CHECK();
   DevBuf keys, vals;
   if (nv > 0) {
     // K1: compaction + Morton codes
@@ -382,6 +377,14 @@ void build_impl(pptk_tree* t, const T* dpts, int64_t n, hipStream_t s) {
   const unsigned split_grid =
       (unsigned)std::min<int64_t>(1024, std::max<int64_t>(1, ceil_div(nv, (int64_t)B * kBlock)));
   int nlev = 1;  // levels [0, nlev) hold nodes
+  // the level table and state come back in one copy per batch of levels
+  std::vector<char> tab(st_off + sizeof(LevelState));
+  const int32_t* hs = (const int32_t*)tab.data();
+  const int32_t* he = hs + (max_levels + 1);
+  auto read_table = [&] {
+    PK_HIP(hipMemcpyAsync(tab.data(), lev.get(), tab.size(), hipMemcpyDeviceToHost, s));
+    PK_HIP(hipStreamSynchronize(s));
+  };
   if (nv > 0) {
     int balanced = 0;
     while (((int64_t)B << balanced) < nv) ++balanced;
@@ -391,23 +394,18 @@ void build_impl(pptk_tree* t, const T* dpts, int64_t n, hipStream_t s) {
       for (int l = l0; l < l1; ++l) {
         split_level_dev_kernel<<<split_grid, kLevelBlock, 0, s>>>(
             keys.as<uint64_t>(), nodes.as<NodeRec>(), lev_s, lev_e, done.as<uint32_t>(), l,
-            max_levels, (int)B, cap, state.as<LevelState>());
+            max_levels, (int)B, cap, state_d);
         PK_LAUNCH_CHECK();
       }
-      int32_t next[2] = {0, 0};
-      PK_HIP(hipMemcpyAsync(&next[0], lev_s + l1, 4, hipMemcpyDeviceToHost, s));
-      PK_HIP(hipMemcpyAsync(&next[1], lev_e + l1, 4, hipMemcpyDeviceToHost, s));
-      PK_HIP(hipStreamSynchronize(s));
-      if (next[1] <= next[0] || l1 >= max_levels) break;
+      read_table();
+      if (he[l1] <= hs[l1] || l1 >= max_levels) break;
       l0 = l1;
     }
+  } else {
+    read_table();
   }
   LevelState st;
-  std::vector<int32_t> hs((size_t)max_levels + 1), he((size_t)max_levels + 1);
-  PK_HIP(hipMemcpyAsync(&st, state.get(), sizeof(st), hipMemcpyDeviceToHost, s));
-  PK_HIP(hipMemcpyAsync(hs.data(), lev_s, hs.size() * 4, hipMemcpyDeviceToHost, s));
-  PK_HIP(hipMemcpyAsync(he.data(), lev_e, he.size() * 4, hipMemcpyDeviceToHost, s));
-  PK_HIP(hipStreamSynchronize(s));
+  std::memcpy(&st, tab.data() + st_off, sizeof(st));
   if (st.overflow) fail(PPTK_E_RUNTIME, "tree build exceeded its node or level bound");
   nlev = 0;
   for (int l = 0; l <= max_levels; ++l)

@@ -23,6 +23,30 @@ struct LevelState {
   unsigned long long nfb;  // batch: rows that took the FP64 eigensolve
 };
 
+// Start of a single-tree build: node 0 = [0, n) with no children, level 0 =
+// {node 0}, levels 1..nlev empty, no split launch done, counter = 1. One
+// block; replaces host copies and memsets that would each be a queue entry.
+static __global__ __launch_bounds__(kLevelBlock) void level_init_kernel(NodeRec* __restrict__ nodes,
+                                                            LevelState* __restrict__ st,
+                                                            int32_t* __restrict__ lev_s,
+                                                            int32_t* __restrict__ lev_e,
+                                                            uint32_t* __restrict__ done,
+                                                            int nlev, int32_t n) {
+  for (int i = threadIdx.x; i <= nlev; i += kLevelBlock) {
+    lev_s[i] = 0;
+    lev_e[i] = i == 0 ? 1 : 0;
+    if (i < nlev) done[i] = 0u;
+  }
+  if (threadIdx.x == 0) {
+    NodeRec root;
+    root.a = 0; root.b = n; root.child = -1; root.parent = -1;
+    nodes[0] = root;
+    LevelState z;
+    z.counter = 1; z.overflow = 0; z.pad[0] = 0; z.pad[1] = 0; z.nfb = 0ull;
+    *st = z;
+  }
+}
+
 // Split every node of level l (the same rule as the single-tree build: the
 // highest differing key bit, or the middle when all keys are equal). Only the
 // blocks the level's width needs take part (the rest return at once, so a

@@ -297,9 +297,8 @@ void exclusive_scan(const Tin* in, Tout* out, int64_t n, hipStream_t s) {
   scan_down_kernel<Tin, Tout><<<dim3((unsigned)tiles), dim3(kScanBlock), 0, s>>>(in, out, n, ts);
   PK_LAUNCH_CHECK();
   PK_HIP(hipMemcpyAsync(out + n, ts + tiles, sizeof(Tout), hipMemcpyDeviceToDevice, s));
-  // `sums` is returned to the pool; callers synchronize the stream before
-  // the pool can hand the block to another stream.
-  PK_HIP(hipStreamSynchronize(s));
+  // `sums` returns to the pool fenced on s (no other stream gets it before
+  // this work is done), so no synchronisation here
 }
 
 template void exclusive_scan<int32_t, int64_t>(const int32_t*, int64_t*, int64_t, hipStream_t);
@@ -379,7 +378,7 @@ void radix_sort_pairs(uint64_t* keys, uint32_t* vals, int64_t n, int end_bit, hi
   SortWorkspace ws;
   ws.reserve(n);
   radix_sort_pairs_async(keys, vals, n, end_bit, ws, s);
-  PK_HIP(hipStreamSynchronize(s));  // the workspace returns to the pool
+  // the workspace returns to the pool fenced on s
 }
 
 void scan_u32_inplace_async(uint32_t* data, int64_t n, hipStream_t s) {

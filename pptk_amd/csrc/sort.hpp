@@ -36,11 +36,12 @@ void radix_sort_pairs_async(uint64_t* keys, uint32_t* vals, int64_t n, int end_b
 void scan_u32_inplace_async(uint32_t* data, int64_t n, hipStream_t s);
 
 // Stable LSD radix sort of (key, value) pairs on key bits [0, end_bit),
-// in place. Synchronizes `s` before returning.
+// in place; asynchronous on `s` (its temporaries return to the pool fenced
+// on s).
 void radix_sort_pairs(uint64_t* keys, uint32_t* vals, int64_t n, int end_bit, hipStream_t s);
 
-// out[0..n] = exclusive prefix sums of in[0..n), out[n] = total.
-// Synchronizes `s` before returning.
+// out[0..n] = exclusive prefix sums of in[0..n), out[n] = total;
+// asynchronous on `s`.
 template <typename Tin, typename Tout>
 void exclusive_scan(const Tin* in, Tout* out, int64_t n, hipStream_t s);
 

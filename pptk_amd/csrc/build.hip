@@ -337,8 +337,9 @@ void build_impl(pptk_tree* t, const T* dpts, int64_t n, hipStream_t s) {
   // Level table: every split launch reads its range from device memory and
   // publishes the next level, so the host reads it back only once per batch
   // of levels to see whether the tree is complete: first after the balanced
-  // depth log2(nv / B) plus 12 (Morton splits are uneven), then every
-  // kLevelBatch levels. A launch past the last level only reads an empty
+  // depth log2(nv / B) plus 20 (Morton splits are uneven; a 10M-point lidar
+  // scene at 192 points per leaf is 36 levels deep), then every kLevelBatch
+  // levels. A launch past the last level only reads an empty
   // range. Depth is bounded by the key bits plus 31 halvings of equal keys.
   constexpr int kLevelBatch = 8;
   const int max_levels = DIM * MortonBits<DIM>::value + 33;
@@ -352,9 +353,8 @@ void build_impl(pptk_tree* t, const T* dpts, int64_t n, hipStream_t s) {
   int32_t* lev_e = lev_s + (max_levels + 1);
   LevelState* state_d = (LevelState*)((char*)lev.get() + st_off);
   // root node, state and level table set by one kernel: no host copies
-  level_init_kernel<<<1, kLevelBlock, 0, s>>>(nodes.as<NodeRec>(), state_d, lev_s,
-                                               lev_e, done.as<uint32_t>(), max_levels,
-                                               (int32_t)nv);
+  level_init_kernel<<<1, kLevelBlock, 0, s>>>(nodes.as<NodeRec>(), state_d, lev_s, lev_e,
+                                               done.as<uint32_t>(), max_levels, (int32_t)nv);
   PK_LAUNCH_CHECK();
   DevBuf keys, vals;
   if (nv > 0) {
@@ -374,8 +374,8 @@ void build_impl(pptk_tree* t, const T* dpts, int64_t n, hipStream_t s) {
     PK_LAUNCH_CHECK();
   }
   // K3: split level by level until a level creates no children.
-  const unsigned split_grid =
-      (unsigned)std::min<int64_t>(1024, std::max<int64_t>(1, ceil_div(nv, (int64_t)B * kBlock)));
+  const unsigned split_grid = (unsigned)std::min<int64_t>(
+      1024, std::max<int64_t>(kSplitProbeLanes / kLevelBlock, ceil_div(nv, (int64_t)B * kBlock)));
   int nlev = 1;  // levels [0, nlev) hold nodes
   // the level table and state come back in one copy per batch of levels
   std::vector<char> tab(st_off + sizeof(LevelState));
@@ -389,7 +389,7 @@ void build_impl(pptk_tree* t, const T* dpts, int64_t n, hipStream_t s) {
     int balanced = 0;
     while (((int64_t)B << balanced) < nv) ++balanced;
     for (int l0 = 0;;) {
-      const int l1 = std::min(l0 == 0 ? std::max(kLevelBatch, balanced + 12) : l0 + kLevelBatch,
+      const int l1 = std::min(l0 == 0 ? std::max(kLevelBatch, balanced + 20) : l0 + kLevelBatch,
                               max_levels);
       for (int l = l0; l < l1; ++l) {
         split_level_dev_kernel<<<split_grid, kLevelBlock, 0, s>>>(
@@ -420,7 +420,8 @@ void build_impl(pptk_tree* t, const T* dpts, int64_t n, hipStream_t s) {
                         hipMemcpyDeviceToDevice, s));
   t->box.reset((size_t)nn * sizeof(BoxRec<D, DIM>));
   t->leafof.reset((size_t)std::max<int64_t>(nv, 1) * 4);
-  if (nv > 0) {
+  {
+    // leaf boxes (the empty tree's root gets the empty box)
     const unsigned g =
         (unsigned)std::min<int64_t>(4096, std::max<int64_t>(1, ceil_div(nn * 8, kLevelBlock)));
     leaf_box_kernel<D, DIM><<<g, kLevelBlock, 0, s>>>(
@@ -428,12 +429,25 @@ void build_impl(pptk_tree* t, const T* dpts, int64_t n, hipStream_t s) {
         t->leafof.as<int32_t>(), nn, nullptr);
     PK_LAUNCH_CHECK();
   }
-  for (int l = nlev - 1; l >= 0; --l) {
+  // Inner boxes bottom-up (the deepest level holds only leaves): a wide level
+  // takes a launch of its own, a run of narrow levels one single-block launch.
+  for (int l = nlev - 2; l >= 0;) {
     const int64_t width = he[l] - hs[l];
-    const unsigned g = (unsigned)std::min<int64_t>(1024, std::max<int64_t>(1, ceil_div(width, kBlock)));
-    box_level_dev_kernel<D, DIM><<<g, kLevelBlock, 0, s>>>(
-        t->node.as<NodeRec>(), t->box.as<BoxRec<D, DIM>>(), lev_s, lev_e, l);
+    if (width > kBoxRunWidth) {
+      const unsigned g =
+          (unsigned)std::min<int64_t>(1024, std::max<int64_t>(1, ceil_div(width, kLevelBlock)));
+      box_level_dev_kernel<D, DIM><<<g, kLevelBlock, 0, s>>>(
+          t->node.as<NodeRec>(), t->box.as<BoxRec<D, DIM>>(), lev_s, lev_e, l);
+      PK_LAUNCH_CHECK();
+      --l;
+      continue;
+    }
+    int lo = l;
+    while (lo > 0 && he[lo - 1] - hs[lo - 1] <= kBoxRunWidth) --lo;
+    box_levels_run_kernel<D, DIM><<<1, kBoxRunBlock, 0, s>>>(
+        t->node.as<NodeRec>(), t->box.as<BoxRec<D, DIM>>(), lev_s, lev_e, l, lo);
     PK_LAUNCH_CHECK();
+    l = lo - 1;
   }
   PK_HIP(hipStreamSynchronize(s));  // temporaries return to the pool
 }

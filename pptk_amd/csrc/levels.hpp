@@ -3,18 +3,20 @@
 //
 // A split launch reads its node range [lev_s[l], lev_e[l]) from device
 // memory, splits every node of the range that holds more than `leaf` points
-// (at the highest key bit where its first and last keys differ, found by
-// binary search, or at the middle when all its keys are equal), appends the
-// children in adjacent pairs at the shared counter, and its last block to
-// finish publishes level l + 1. One launch then computes every leaf's box,
-// and the box launches combine child boxes bottom-up over the same level
-// table. No host read is needed between levels.
+// (at the highest key bit where its first and last keys differ, found by a
+// search over the sorted keys, or at the middle when all its keys are equal),
+// appends the children in adjacent pairs at the shared counter, and its last
+// block to finish publishes level l + 1. One launch then computes every
+// leaf's box, and the box launches combine child boxes bottom-up over the
+// same level table. No host read is needed between levels.
 #pragma once
 #include "common.hpp"
 
 namespace pk {
 
 constexpr int kLevelBlock = 256;
+// lanes a split launch spends on the searches of a narrow level
+constexpr int64_t kSplitProbeLanes = 16384;
 
 struct LevelState {
   int32_t counter;    // next free node id
@@ -47,11 +49,52 @@ static __global__ __launch_bounds__(kLevelBlock) void level_init_kernel(NodeRec*
   }
 }
 
+// First index p in [lo, hi] whose key has `bit` set, for sorted keys that
+// agree above `bit` and with keys[hi] having it set. One lane (g = 1) does a
+// binary search. A group of g > 1 lanes (g a power of two <= 64, aligned in
+// its wave, converged) probes lo + j * span / g for j < g and keeps the part
+// where the bit turns on, so a range of r keys takes about log_g(r) dependent
+// loads instead of log_2(r). Every lane of the group returns the same p.
+__device__ __forceinline__ int32_t first_set_bit_index(const uint64_t* __restrict__ keys,
+                                                       int32_t lo, int32_t hi, int bit,
+                                                       int lg, int sub, int gbase) {
+  if (lg == 0) {
+    while (lo < hi) {
+      const int32_t mid = lo + ((hi - lo) >> 1);
+      if ((keys[mid] >> bit) & 1ull) hi = mid;
+      else lo = mid + 1;
+    }
+    return lo;
+  }
+  const int g = 1 << lg;
+  const uint64_t gmask = g == 64 ? ~0ull : (((1ull << g) - 1ull) << gbase);
+  while (lo < hi) {
+    const int64_t span = (int64_t)hi - lo;
+    // probe j: lo + j * span / g, in [lo, hi) and non-decreasing in j
+    const int32_t q = lo + (int32_t)(((int64_t)sub * span) >> lg);
+    const uint64_t m = __ballot((keys[q] >> bit) & 1ull) & gmask;
+    if (m) {
+      const int j = __ffsll((unsigned long long)m) - 1 - gbase;  // first probe with the bit
+      // probe j - 1 (a different, lower index) lacks the bit
+      const int32_t qj = lo + (int32_t)(((int64_t)j * span) >> lg);
+      if (j > 0) lo = lo + (int32_t)(((int64_t)(j - 1) * span) >> lg) + 1;
+      hi = qj;
+    } else {
+      lo = lo + (int32_t)(((int64_t)(g - 1) * span) >> lg) + 1;
+    }
+  }
+  return lo;
+}
+
 // Split every node of level l (the same rule as the single-tree build: the
-// highest differing key bit, or the middle when all keys are equal). Only the
-// blocks the level's width needs take part (the rest return at once, so a
-// narrow level costs no more than a few blocks); the last of them to finish
-// publishes level l+1 = the pairs this launch appended.
+// highest differing key bit, or the middle when all keys are equal). A
+// narrow level gives each node a group of up to 64 lanes for the split
+// search, as many as keep width * lanes within kSplitProbeLanes (the top and
+// bottom levels of a tree are narrow, and their searches would otherwise be
+// long chains of dependent loads); a wide level gives each node one lane.
+// Only the blocks the level needs take part (the rest return at once); the
+// last of them to finish publishes level l+1 = the pairs this launch
+// appended.
 static __global__ __launch_bounds__(kLevelBlock) void split_level_dev_kernel(const uint64_t* __restrict__ keys,
                                                                  NodeRec* __restrict__ nodes,
                                                                  int32_t* __restrict__ lev_s,
@@ -62,11 +105,18 @@ static __global__ __launch_bounds__(kLevelBlock) void split_level_dev_kernel(con
                                                                  LevelState* __restrict__ st) {
   const int32_t s = lev_s[l], e = lev_e[l];
   const int64_t width = (int64_t)e - s;
-  const int64_t want = width > 0 ? (width + kLevelBlock - 1) / kLevelBlock : 1;
+  int lg = 0;
+  while (lg < 6 && width * (2 << lg) <= kSplitProbeLanes) ++lg;
+  const int g = 1 << lg;
+  const int64_t want = width > 0 ? (width * g + kLevelBlock - 1) / kLevelBlock : 1;
   const unsigned used = (unsigned)(want < (int64_t)gridDim.x ? want : (int64_t)gridDim.x);
   if (blockIdx.x >= used) return;
-  const int64_t stride = (int64_t)used * kLevelBlock;
-  for (int64_t i = s + (int64_t)blockIdx.x * kLevelBlock + threadIdx.x; i < e; i += stride) {
+  const int lane = (int)(threadIdx.x & 63);
+  const int sub = lane & (g - 1);
+  const int gbase = lane - sub;
+  const int64_t groups = (int64_t)used * (kLevelBlock / g);
+  // every lane of a group takes the same nodes and the same branches
+  for (int64_t i = s + ((int64_t)blockIdx.x * kLevelBlock + threadIdx.x) / g; i < e; i += groups) {
     const int32_t a = nodes[i].a, b = nodes[i].b;
     if (b - a <= leaf) continue;  // child stays -1
     const uint64_t x = keys[a] ^ keys[b - 1];
@@ -75,14 +125,9 @@ static __global__ __launch_bounds__(kLevelBlock) void split_level_dev_kernel(con
       split = a + (b - a) / 2;
     } else {
       const int bit = 63 - __clzll((long long)x);
-      int32_t lo = a + 1, hi = b - 1;
-      while (lo < hi) {
-        const int32_t mid = lo + ((hi - lo) >> 1);
-        if ((keys[mid] >> bit) & 1ull) hi = mid;
-        else lo = mid + 1;
-      }
-      split = lo;
+      split = first_set_bit_index(keys, a + 1, b - 1, bit, lg, sub, gbase);
     }
+    if (sub != 0) continue;
     const int c = atomicAdd(&st->counter, 2);
     if ((int64_t)c + 2 > cap || l + 1 >= nlev) {
       st->overflow = 1;
@@ -163,8 +208,25 @@ static __global__ __launch_bounds__(kLevelBlock) void leaf_box_kernel(const Poin
   }
 }
 
-// Boxes of the inner nodes of level l from their children's boxes (leaves
-// are done by leaf_box_kernel); run deepest level first.
+// Box of inner node i from its children's boxes (leaves keep theirs).
+template <typename D, int DIM>
+__device__ __forceinline__ void combine_child_boxes(const NodeRec* __restrict__ nodes,
+                                                    BoxRec<D, DIM>* __restrict__ box, int64_t i) {
+  const int32_t c = nodes[i].child;
+  if (c < 0) return;  // leaf boxes come from leaf_box_kernel
+  const BoxRec<D, DIM> x = box[c];
+  const BoxRec<D, DIM> y = box[c + 1];
+  BoxRec<D, DIM> b;
+#pragma unroll
+  for (int d = 0; d < PadDim<DIM>::value; ++d) {
+    b.lo[d] = d < DIM ? (x.lo[d] < y.lo[d] ? x.lo[d] : y.lo[d]) : D(0);
+    b.hi[d] = d < DIM ? (x.hi[d] > y.hi[d] ? x.hi[d] : y.hi[d]) : D(0);
+  }
+  box[i] = b;
+}
+
+// Boxes of the inner nodes of level l from their children's boxes; run
+// deepest level first.
 template <typename D, int DIM>
 static __global__ __launch_bounds__(kLevelBlock) void box_level_dev_kernel(const NodeRec* __restrict__ nodes,
                                                                BoxRec<D, DIM>* __restrict__ box,
@@ -173,26 +235,28 @@ static __global__ __launch_bounds__(kLevelBlock) void box_level_dev_kernel(const
                                                                int l) {
   const int32_t s = lev_s[l], e = lev_e[l];
   const int64_t stride = (int64_t)gridDim.x * kLevelBlock;
-  for (int64_t i = s + (int64_t)blockIdx.x * kLevelBlock + threadIdx.x; i < e; i += stride) {
-    const NodeRec nr = nodes[i];
-    BoxRec<D, DIM> b;
-#pragma unroll
-    for (int d = 0; d < PadDim<DIM>::value; ++d) {
-      b.lo[d] = d < DIM ? pos_inf<D>() : D(0);
-      b.hi[d] = d < DIM ? -pos_inf<D>() : D(0);
-    }
-    if (nr.child < 0) {
-      continue;  // leaf boxes come from leaf_box_kernel
-    } else {
-      const BoxRec<D, DIM> x = box[nr.child];
-      const BoxRec<D, DIM> y = box[nr.child + 1];
-#pragma unroll
-      for (int d = 0; d < DIM; ++d) {
-        b.lo[d] = x.lo[d] < y.lo[d] ? x.lo[d] : y.lo[d];
-        b.hi[d] = x.hi[d] > y.hi[d] ? x.hi[d] : y.hi[d];
-      }
-    }
-    box[i] = b;
+  for (int64_t i = s + (int64_t)blockIdx.x * kLevelBlock + threadIdx.x; i < e; i += stride)
+    combine_child_boxes<D, DIM>(nodes, box, i);
+}
+
+// The same for a run of narrow levels l_hi, l_hi - 1, ..., l_lo in ONE block:
+// a barrier between levels takes the place of a launch per level (the block's
+// waves share a compute unit, so their box writes are visible to each other
+// after the barrier).
+constexpr int kBoxRunBlock = 1024;
+constexpr int64_t kBoxRunWidth = 4 * kBoxRunBlock;  // widest level a run takes
+
+template <typename D, int DIM>
+static __global__ __launch_bounds__(kBoxRunBlock) void box_levels_run_kernel(const NodeRec* __restrict__ nodes,
+                                                                 BoxRec<D, DIM>* __restrict__ box,
+                                                                 const int32_t* __restrict__ lev_s,
+                                                                 const int32_t* __restrict__ lev_e,
+                                                                 int l_hi, int l_lo) {
+  for (int l = l_hi; l >= l_lo; --l) {
+    const int32_t s = lev_s[l], e = lev_e[l];
+    for (int64_t i = s + threadIdx.x; i < e; i += kBoxRunBlock)
+      combine_child_boxes<D, DIM>(nodes, box, i);
+    __syncthreads();
   }
 }
 

@@ -73,6 +73,8 @@ def algorithmic_bytes(n, k, m_rnear, rnear_total):
         "rnear_fill": m_rnear * (3 * s + 16) + 4 * rnear_total,
         # read point, write the unit normal
         "normals": n * (3 * s + 3 * s),
+        # one search: read point, write the kNN row + count and the normal
+        "knn_normals": n * (3 * s + 4 * k + 4 + 3 * s),
     }
 
 
@@ -82,6 +84,7 @@ PHASE_KERNEL = {
     "rnear_count": "rnear_packet_kernel<float, 3, false, false>",
     "rnear_fill": "rnear_packet_kernel<float, 3, true, false>",
     "normals": "normals_coop_kernel<16, 1, 1>",
+    "knn_normals": "knn_normals_coop_kernel<16, 1, 1>",
 }
 
 
@@ -318,39 +321,42 @@ def bench_lidar(args):
     q_d = torch.from_numpy(qpts).to(dev)
     prec = capi.PREC_MIXED if args.precision == "mixed" else capi.PREC_FP64
     pipe = pipeline.LidarStep(pts_d, q_d, k=k, dmax=dmax, leaf=args.leaf, precision=prec)
-    phases = list(pipeline.LidarStep.PHASES)
 
-    def step(record):
-        ev = None
-        if record is not None:
+    def timed(p, warmup, steps, say):
+        """(ms per step by the host clock, mean phase ms by events)."""
+        phases = list(p.PHASES)
+        rec = {}
+        for _ in range(warmup):
+            p.run(None)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(steps):
             ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(phases) + 1)]
-        pipe.run(ev)
-        if record is not None:
-            pipe.stream.synchronize()
-            for i, p in enumerate(phases):
-                record.setdefault(p, []).append(ev[i].elapsed_time(ev[i + 1]))
+            p.run(ev)
+            p.stream.synchronize()
+            for j, name in enumerate(phases):
+                rec.setdefault(name, []).append(ev[j].elapsed_time(ev[j + 1]))
+            if say:
+                log("bench: step %d done" % i)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        return wall * 1e3 / steps, {q: float(np.mean(v)) for q, v in rec.items()}
 
-    log("bench: warmup %d" % args.warmup)
-    for _ in range(args.warmup):
-        step(None)
-    torch.cuda.synchronize()
-    rec = {}
-    log("bench: timing %d steps" % args.steps)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        step(rec)
-        log("bench: step %d done" % i)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_step = wall * 1e3 / args.steps
-    ph = {p: float(np.mean(v)) for p, v in rec.items()}
+    log("bench: warmup %d, timing %d steps" % (args.warmup, args.steps))
+    ms_step, ph = timed(pipe, args.warmup, args.steps, True)
+    # the same step with the two separate calls (two searches per point)
+    log("bench: the step with separate kNN and normals calls")
+    sep = pipeline.LidarStep(pts_d, q_d, k=k, dmax=dmax, leaf=args.leaf, precision=prec,
+                             fused=False)
+    sep_ms, sep_ph = timed(sep, 2, min(args.steps, 5), False)
+    del sep
 
     ab = algorithmic_bytes(n, k, m, pipe.rtotal)
-    dom = max(PHASE_KERNEL, key=lambda p: ph[p])
+    dom = max((p for p in PHASE_KERNEL if p in ph), key=lambda p: ph[p])
     achieved = ab[dom] / (ph[dom] * 1e-3) / 1e9
     traffic = measured_traffic(PHASE_KERNEL[dom])
-    plain_ms = ph["build"] + ph["knn"] + ph["normals"]
+    # the single-cloud path for kNN + normals of every point of the cloud
+    plain_ms = ph["build"] + ph["knn_normals"]
     out = {
         "metric": METRIC,
         "value": 2 * n / (ms_step * 1e-3),
@@ -372,8 +378,14 @@ def bench_lidar(args):
             "leaf_size": pipe.leaf_size,
         },
         "phases_ms": ph,
-        "knn_qps": n / (ph["knn"] * 1e-3),
-        "normals_ps": n / (ph["normals"] * 1e-3),
+        "knn_normals_points_per_s": n / (ph["knn_normals"] * 1e-3),
+        "separate_calls": {
+            "what": "the same step with pptk_knn then pptk_normals (a search each) in place "
+                    "of pptk_knn_normals; outputs bit-identical",
+            "ms_per_step": sep_ms, "phases_ms": sep_ph,
+            "knn_qps": n / (sep_ph["knn"] * 1e-3),
+            "normals_ps": n / (sep_ph["normals"] * 1e-3),
+        },
         "rnear_qps": m / ((ph["rnear_count"] + ph["rnear_fill"]) * 1e-3),
         "build_pts_per_s": n / (ph["build"] * 1e-3),
         "plain_10m_points_per_s": n / (plain_ms * 1e-3),

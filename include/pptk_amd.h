@@ -236,6 +236,24 @@ pptk_status pptk_normals(const pptk_tree* tree, int k, double r, const int32_t* 
                          void* out_evecs, void* out_evals, int32_t* out_sizes,
                          int64_t* out_num_fp64, pptk_mem mem, void* stream);
 
+/* Self-query kNN and PCA normals of one tree from ONE search per point. No
+ * single counterpart in pptk, where a caller that wants both makes two calls,
+ * kdtree._query(tree, k=k, dmax=dmax) (kdtree_wrapper.cpp:281-304, the
+ * queries=None self query) and estimate_normals(P, nk, r)
+ * (estimate_normals.cpp:64-111, which builds and searches a tree of its own):
+ * this call writes exactly what
+ *   pptk_knn(tree, {PPTK_Q_ALL}, k, dmax, out_idx, out_cnt, out_dist, mem, stream)
+ *   pptk_normals(tree, nk, r, NULL, 0, num_eigen, precision, out_evecs,
+ *                out_evals, out_sizes, out_num_fp64, mem, stream)
+ * write, with the arguments, output shapes and errors of those two calls
+ * (k >= 1 and nk >= 1 here). Float32 trees with max(k, nk) <= 32 search
+ * once; other trees run the two calls' searches. */
+pptk_status pptk_knn_normals(const pptk_tree* tree, int k, double dmax, int nk, double r,
+                             int num_eigen, pptk_precision precision, int32_t* out_idx,
+                             int32_t* out_cnt, void* out_dist, void* out_evecs,
+                             void* out_evals, int32_t* out_sizes, int64_t* out_num_fp64,
+                             pptk_mem mem, void* stream);
+
 /* ------------------------------------------------------------------------
  * Batched clouds. No counterpart in pptk, where each cloud is one
  * kdtree._build plus kdtree._query(None, k) plus estimate_normals call

@@ -36,7 +36,8 @@ EXPORTED = [
     "pptk_last_error", "pptk_device_count", "pptk_trim", "pptk_tree_build", "pptk_tree_free",
     "pptk_tree_check", "pptk_tree_get_info", "pptk_tree_export", "pptk_knn", "pptk_knn_noalloc",
     "pptk_rnear_count",
-    "pptk_rnear_fill", "pptk_normals", "pptk_sort_pairs_u64", "pptk_exclusive_scan_i32",
+    "pptk_rnear_fill", "pptk_normals", "pptk_knn_normals", "pptk_sort_pairs_u64",
+    "pptk_exclusive_scan_i32",
     "pptk_batch_create", "pptk_batch_run", "pptk_batch_get_info", "pptk_batch_free",
     "pptk_tree_serialize", "pptk_tree_deserialize",
 ]
@@ -114,6 +115,9 @@ def lib():
     L.pptk_normals.restype = ci
     L.pptk_normals.argtypes = [vp, ci, cd, vp, i64, ci, ci, vp, vp, vp, ctypes.POINTER(i64), ci,
                                vp]
+    L.pptk_knn_normals.restype = ci
+    L.pptk_knn_normals.argtypes = [vp, ci, cd, ci, cd, ci, ci, vp, vp, vp, vp, vp, vp,
+                                   ctypes.POINTER(i64), ci, vp]
     L.pptk_sort_pairs_u64.restype = ci
     L.pptk_sort_pairs_u64.argtypes = [vp, vp, i64, ci, ci]
     L.pptk_exclusive_scan_i32.restype = ci
@@ -338,6 +342,29 @@ def normals(tree, k, r, rows=None, num_eigen=1, precision=PREC_AUTO, want_evals=
                              0 if rr is None else rr.shape[0], int(num_eigen), int(precision),
                              _p(ev), _p(ew), _p(sz), ctypes.byref(nfb), MEM_HOST, stream))
     return ev, ew, sz, int(nfb.value)
+
+
+def knn_normals(tree, k, nk, dmax=np.inf, r=np.inf, num_eigen=1, precision=PREC_AUTO,
+                with_dist=False, want_evals=True, want_sizes=True, stream=None):
+    """Host-memory self-query kNN + normals from one search (pptk_knn_normals):
+    (idx, cnt, dist or None, evecs, evals, sizes, num_fp64), each as knn()
+    with queries=None and normals() with rows=None return it."""
+    inf = tree.info
+    dt = np.float64 if inf.dtype == F64 else np.float32
+    m = inf.num_valid
+    idx = np.empty((m, k), np.int32)
+    cnt = np.empty(m, np.int32)
+    dist = np.empty((m, k), tree.dist_dtype) if with_dist else None
+    n = inf.num_input
+    ev = np.empty((n, 3, 3) if num_eigen == 3 else (n, 3), dt)
+    ew = (np.empty((n, 3) if num_eigen == 3 else (n,), dt)) if want_evals else None
+    sz = np.empty(n, np.int32) if want_sizes else None
+    nfb = ctypes.c_int64()
+    check(lib().pptk_knn_normals(ctypes.c_void_p(tree.handle), int(k), float(dmax), int(nk),
+                                 float(r), int(num_eigen), int(precision), _p(idx), _p(cnt),
+                                 _p(dist), _p(ev), _p(ew), _p(sz), ctypes.byref(nfb), MEM_HOST,
+                                 stream))
+    return idx, cnt, dist, ev, ew, sz, int(nfb.value)
 
 
 class Batch:

@@ -368,11 +368,13 @@ __global__ __launch_bounds__(kBlock) void roots_kernel(const uint32_t* __restric
 }
 
 // Fused self-query kNN + PCA normals, one lane per tree position. The search
-// finds the kk = max(k, nk - 1) nearest other points of the cloud; the kNN
-// row is the first k of them. The normals neighbourhood (nk nearest
-// including the point itself, as estimate_normals searches) is that list
-// with the point merged in at its (0, own index) rank, truncated to nk: the
-// same set, summed in the same order, as a separate search would give.
+// finds the kk = max(k, nk) nearest other points of the cloud; the kNN row is
+// the first k of them. The normals neighbourhood (nk nearest including the
+// point itself, as estimate_normals searches) is that list with the point
+// merged in at its (0, own index) rank, truncated to nk: the same set, summed
+// in the same order, as a separate search would give. nk others (not nk - 1)
+// are needed when nk or more points coincide with the point at lower indices:
+// then the point itself is not among its own nk nearest.
 template <typename D, int DIM, int K, int PREC, bool NORM>
 // Float forests with K <= 16 hold 7 waves per SIMD (<= 72 VGPRs): at 64
 // VGPRs the top-k merges spill, and without a bound the compiler's budget
@@ -701,7 +703,7 @@ pptk_status api_batch_create(const int64_t* sizes, int64_t C, int dim, pptk_dtyp
     b->nlev = b->shift + 33;
     b->k = prm->k;
     b->nk = prm->normals_k;
-    b->kk = std::max(b->k, b->nk - 1);
+    b->kk = std::max(b->k, b->nk);
     b->dmax = prm->dmax;
     b->prec = prm->precision == PPTK_PREC_AUTO
                   ? (dtype == PPTK_F64 ? PPTK_PREC_FP64 : PPTK_PREC_MIXED)

@@ -158,6 +158,102 @@ __global__ __launch_bounds__(kBlock, K <= 16 ? 8 : 1) void normals_coop_kernel(T
   count_fallbacks(fb, nfb);
 }
 
+// The lane's index in its wave from an asm block, which the compiler cannot
+// merge with an earlier lane id: a value rebuilt from it after a search is
+// not kept in a register (or spilled) through the search.
+__device__ __forceinline__ int32_t fresh_lane_id() {
+  int32_t v;
+  asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(v));
+  return v;
+}
+
+// Self-query kNN rows and normals of a float tree from ONE search per point
+// (pptk_knn_normals). The wave-cooperative search finds the kk = max(k, nk)
+// nearest OTHER points within the larger of the two bounds, in (d, index)
+// order. The kNN row is the first k of them with d < dmax^2: the k nearest
+// with d < dmax^2, as pptk_knn writes for PPTK_Q_ALL (rows 0 .. m-1 by
+// original index). The normals neighbourhood, the nk nearest including the
+// point itself with d < r^2 as normals_coop_kernel searches, is that list
+// with the point merged in at its (0, own index) rank, cut to d < r^2 and to
+// nk entries: the same set in the same order, so the same sums. nk others
+// (not nk - 1) are needed when nk or more points coincide with the point at
+// lower indices: then the point itself is not among its own nk nearest.
+template <int K, int PREC, int NE>
+__global__ __launch_bounds__(kBlock, K <= 16 ? 8 : 1) void knn_normals_coop_kernel(
+    TreeView<float, 3> tv, int64_t m, int k, int kk, int nk, float r2s, float r2k, float r2n,
+    int32_t* __restrict__ out_idx, int32_t* __restrict__ out_cnt, float* __restrict__ out_dist,
+    float* __restrict__ evecs, float* __restrict__ evals, int32_t* __restrict__ sizes,
+    unsigned long long* __restrict__ nfb) {
+  __shared__ uint64_t lds[kBlock / 64 * coop_wave_bytes<K, 3, K>() / 8];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  // tree positions fit int32; the lane's own is rebuilt from the wave's
+  // first after the search instead of being held through it
+  const int32_t t0 = (int32_t)blockIdx.x * kBlock + wave * 64;
+  const int32_t t = t0 + (int32_t)__lane_id();
+  float q[3] = {0.0f, 0.0f, 0.0f};
+  int32_t self = -1;
+  if (t < m) {
+    const PointRec<float, 3> r = tv.pts[t];
+    self = r.orig;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) q[d] = r.c[d];
+  }
+  TopK<float, K, false> st;
+  st.init(kk);
+  uint64_t* wl = lds + wave * (coop_wave_bytes<K, 3, K>() / 8);
+  // the row-broadcast reductions would spill at K 16 (as in knn_coop_kernel)
+  knn_coop_self<3, K, K != 16>(tv, self >= 0, t, self, q, r2s, kk, true, st, wl);
+  if (self < 0) return;
+  const int32_t me = t0 + fresh_lane_id();
+  if (self < m) {
+    int32_t c = 0;
+    int32_t* orow = out_idx + (int64_t)self * k;
+#pragma unroll
+    for (int s = 0; s < K; ++s) {
+      const int j = s - (K - kk);
+      if (j >= 0 && j < k) {
+        const float ds = st.dist(s);
+        const bool f = st.found(s) && ds < r2k;
+        orow[j] = f ? st.idx(s) : -1;
+        if (out_dist) out_dist[(int64_t)self * k + j] = f ? ds : pos_inf<float>();
+        c += f ? 1 : 0;
+      }
+    }
+    out_cnt[self] = c;
+  }
+  // neighbourhood positions in order, the point itself at its rank, staged in
+  // this lane's LDS buffer (free after the search)
+  int32_t* pos = coop_positions<3>(wl);
+  bool pending = 0.0f < r2n;
+  int n = 0;
+#pragma unroll
+  for (int s = 0; s < K; ++s) {
+    if (s >= K - kk && st.found(s) && st.dist(s) < r2n) {
+      if (pending && key_less(0.0f, self, st.dist(s), st.idx(s))) {
+        if (n < nk) pos[n * 64] = me;
+        n += n < nk ? 1 : 0;
+        pending = false;
+      }
+      if (n < nk) pos[n * 64] = tv.fwd[st.idx(s)];
+      n += n < nk ? 1 : 0;
+    }
+  }
+  if (pending && n < nk) {
+    pos[n * 64] = me;
+    ++n;
+  }
+  bool fb = false;
+  if (n == 0) {
+    write_nan<float, NE>(self, evecs, evals, sizes);
+  } else {
+    auto each = [&](auto f) {
+      for (int j = 0; j < n; ++j) f(pos[j * 64]);
+    };
+    fb = pca_row<float, PREC, NE>(tv, q, n, each, self, evecs, evals, sizes);
+  }
+  count_fallbacks(fb, nfb);
+}
+
 template <typename D, int PREC, int NE>
 __global__ __launch_bounds__(kBlock) void normals_heap_kernel(TreeView<D, 3> tv,
                                                               const int32_t* __restrict__ rows,
@@ -419,6 +515,82 @@ void normals_run(const pptk_tree* t, int k, double r, const int32_t* rows, int64
   if (out_num_fp64) *out_num_fp64 = (int64_t)nf;
 }
 
+// pptk_knn_normals on a float 3-D tree with max(k, nk) <= 32: the outputs of
+// pptk_knn(PPTK_Q_ALL) and pptk_normals(rows = NULL) from one launch.
+void knn_normals_run(const pptk_tree* t, int k, double dmax, int nk, double r, int num_eigen,
+                     pptk_precision prec, int32_t* out_idx, int32_t* out_cnt, void* out_dist,
+                     void* out_evecs, void* out_evals, int32_t* out_sizes,
+                     int64_t* out_num_fp64, pptk_mem mem, hipStream_t s) {
+  const int64_t m = t->n_valid;      // kNN rows, and the points searched from
+  const int64_t nrows = t->n_input;  // normals rows
+  DevOutput oi, oc, od, oe, ov, os;
+  oi.init(out_idx, (size_t)m * k * 4, mem);
+  oc.init(out_cnt, (size_t)m * 4, mem);
+  od.init(out_dist, (size_t)m * k * 4, mem);
+  oe.init(out_evecs, (size_t)nrows * 3 * num_eigen * 4, mem);
+  ov.init(out_evals, (size_t)nrows * num_eigen * 4, mem);
+  os.init(out_sizes, (size_t)nrows * 4, mem);
+  DevBuf nfb(8);
+  PK_HIP(hipMemsetAsync(nfb.get(), 0, 8, s));
+  float* evecs = (float*)oe.dptr;
+  float* evals = (float*)ov.dptr;
+  int32_t* sizes = (int32_t*)os.dptr;
+  if (m < nrows) {
+    // kNN rows of NaN/inf points get no lane (pre-filled empty, as pptk_knn);
+    // their normals rows are NaN / size 0 (as pptk_normals)
+    if (m > 0) {
+      fill_async<int32_t>((int32_t*)oi.dptr, m * k, -1, s);
+      fill_async<int32_t>((int32_t*)oc.dptr, m, 0, s);
+      if (od.dptr) fill_async<float>((float*)od.dptr, m * k, pos_inf<float>(), s);
+    }
+    const int32_t* fwd = t->fwd.as<const int32_t>();
+    if (num_eigen == 1)
+      nan_rows_kernel<float, 1><<<grid_of(nrows), kBlock, 0, s>>>(fwd, nrows, evecs, evals, sizes);
+    else
+      nan_rows_kernel<float, 3><<<grid_of(nrows), kBlock, 0, s>>>(fwd, nrows, evecs, evals, sizes);
+    PK_LAUNCH_CHECK();
+  }
+  const float rk = (float)dmax, rn = (float)r;
+  const float r2k = rk * rk, r2n = rn * rn;
+  const float r2s = r2k > r2n ? r2k : r2n;
+  const int kk = std::max(k, nk);
+  if (m > 0) {
+    const TreeView<float, 3> tv = tree_view<float, 3>(t);
+    auto launch = [&](auto kc, auto pc, auto ec) {
+      constexpr int K = decltype(kc)::value;
+      constexpr int P = decltype(pc)::value;
+      constexpr int NE = decltype(ec)::value;
+      knn_normals_coop_kernel<K, P, NE><<<grid_of(m), kBlock, 0, s>>>(
+          tv, m, k, kk, nk, r2s, r2k, r2n, (int32_t*)oi.dptr, (int32_t*)oc.dptr,
+          (float*)od.dptr, evecs, evals, sizes, nfb.as<unsigned long long>());
+      PK_LAUNCH_CHECK();
+    };
+    auto by_k = [&](auto pc, auto ec) {
+      if (kk <= 4) launch(std::integral_constant<int, 4>(), pc, ec);
+      else if (kk <= 8) launch(std::integral_constant<int, 8>(), pc, ec);
+      else if (kk <= 16) launch(std::integral_constant<int, 16>(), pc, ec);
+      else launch(std::integral_constant<int, 32>(), pc, ec);
+    };
+    using Mixed = std::integral_constant<int, PPTK_PREC_MIXED>;
+    using F64 = std::integral_constant<int, PPTK_PREC_FP64>;
+    using One = std::integral_constant<int, 1>;
+    using Three = std::integral_constant<int, 3>;
+    const bool mixed = prec == PPTK_PREC_MIXED;
+    if (num_eigen == 1) mixed ? by_k(Mixed(), One()) : by_k(F64(), One());
+    else mixed ? by_k(Mixed(), Three()) : by_k(F64(), Three());
+  }
+  oi.finish(s);
+  oc.finish(s);
+  od.finish(s);
+  oe.finish(s);
+  ov.finish(s);
+  os.finish(s);
+  unsigned long long nf = 0;
+  PK_HIP(hipMemcpyAsync(&nf, nfb.get(), 8, hipMemcpyDeviceToHost, s));
+  PK_HIP(hipStreamSynchronize(s));
+  if (out_num_fp64) *out_num_fp64 = (int64_t)nf;
+}
+
 }  // namespace
 }  // namespace pk
 
@@ -449,4 +621,49 @@ extern "C" pptk_status pptk_normals(const pptk_tree* tree, int k, double r, cons
       pk::normals_run<float>(tree, k, r, rows, m, num_eigen, prec, out_evecs, out_evals,
                              out_sizes, out_num_fp64, mem, s);
   });
+}
+
+namespace pk {
+pptk_status api_knn(const pptk_tree* tree, const pptk_queries* q, int k, double dmax,
+                    int32_t* out_idx, int32_t* out_cnt, void* out_dist, pptk_mem out_mem,
+                    void* stream);
+}  // namespace pk
+
+extern "C" pptk_status pptk_knn_normals(const pptk_tree* tree, int k, double dmax, int nk,
+                                        double r, int num_eigen, pptk_precision precision,
+                                        int32_t* out_idx, int32_t* out_cnt, void* out_dist,
+                                        void* out_evecs, void* out_evals, int32_t* out_sizes,
+                                        int64_t* out_num_fp64, pptk_mem mem, void* stream) {
+  bool fused = false;
+  const pptk_status st = pk::guard([&] {
+    pk::check_tree(tree);
+    if (k < 1) pk::fail(PPTK_E_VALUE, "k must be positive (got %d)", k);
+    if (!(dmax >= 0.0)) pk::fail(PPTK_E_VALUE, "dmax must be non-negative (got %g)", dmax);
+    if (nk < 1) pk::fail(PPTK_E_VALUE, "normals k must be positive (got %d)", nk);
+    if (!(r > 0.0)) pk::fail(PPTK_E_VALUE, "r must be positive");
+    if (tree->dim != 3) pk::fail(PPTK_E_VALUE, "normals need 3-D points (tree dim = %d)", tree->dim);
+    if (tree->dtype != PPTK_F32 && tree->dtype != PPTK_F64)
+      pk::fail(PPTK_E_TYPE, "points must be float32 or float64");
+    if (num_eigen != 1 && num_eigen != 3) pk::fail(PPTK_E_VALUE, "num_eigen must be 1 or 3");
+    if (!out_idx || !out_cnt) pk::fail(PPTK_E_VALUE, "null kNN output");
+    if (!out_evecs) pk::fail(PPTK_E_VALUE, "null eigenvector output");
+    pptk_precision prec = precision;
+    if (prec == PPTK_PREC_AUTO) prec = tree->dtype == PPTK_F64 ? PPTK_PREC_FP64 : PPTK_PREC_MIXED;
+    if (prec != PPTK_PREC_MIXED && prec != PPTK_PREC_FP64)
+      pk::fail(PPTK_E_VALUE, "unknown precision mode %d", (int)precision);
+    if (tree->dist != PPTK_F32 || std::max(k, nk) > 32 || tree->nroots > 1) return;
+    fused = true;
+    hipStream_t s = pk::pick_stream(stream);
+    pk::knn_normals_run(tree, k, dmax, nk, r, num_eigen, prec, out_idx, out_cnt, out_dist,
+                        out_evecs, out_evals, out_sizes, out_num_fp64, mem, s);
+  });
+  if (st != PPTK_OK || fused) return st;
+  // the two searches of the separate calls (float64 trees, k or nk > 32)
+  pptk_queries all{};
+  all.kind = PPTK_Q_ALL;
+  const pptk_status sk =
+      pk::api_knn(tree, &all, k, dmax, out_idx, out_cnt, out_dist, mem, stream);
+  if (sk != PPTK_OK) return sk;
+  return pptk_normals(tree, nk, r, nullptr, 0, num_eigen, precision, out_evecs, out_evals,
+                      out_sizes, out_num_fp64, mem, stream);
 }

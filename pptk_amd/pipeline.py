@@ -10,6 +10,9 @@ with every output left in device memory (torch tensors):
      kdtree-impl.h:1008-1026): CSR offsets (m+1,) int64 + lists int32;
   4. PCA normals of every point on the resident tree
      (estimate_normals, estimate_normals.cpp:64-111): (n, 3).
+By default steps 2 and 4 are one call, pptk_knn_normals, which writes both
+outputs from one search per point (bit for bit what the two separate calls
+write); fused=False makes the two separate calls.
 
 `bench.py` times `LidarStep.run` and the full-size parity test checks its
 outputs, so the numbers and the parity claims cover the same calls.
@@ -28,10 +31,11 @@ class LidarStep:
     points on the GPU (or None to skip r-near). Outputs: idx_d, cnt_d, off_d,
     ridx_d (the first `rtotal` entries are valid), nrm_d."""
 
-    PHASES = ("build", "knn", "rnear_count", "rnear_fill", "normals")
+    PHASES_FUSED = ("build", "knn_normals", "rnear_count", "rnear_fill")
+    PHASES_SEPARATE = ("build", "knn", "rnear_count", "rnear_fill", "normals")
 
     def __init__(self, pts_d, q_d, k=16, dmax=0.1, normals_k=None, leaf=0,
-                 precision=capi.PREC_MIXED, stream=None):
+                 precision=capi.PREC_MIXED, stream=None, fused=True):
         import torch
         self.torch = torch
         self.dev = pts_d.device
@@ -41,6 +45,8 @@ class LidarStep:
         self.dmax = float(np.float32(dmax))  # kdtree._query rounds dmax through float32
         self.leaf = int(leaf)
         self.precision = precision
+        self.fused = bool(fused)
+        self.PHASES = self.PHASES_FUSED if self.fused else self.PHASES_SEPARATE
         self.pts_d = pts_d
         self.q_d = q_d
         self.m = 0 if q_d is None else int(q_d.shape[0])
@@ -72,6 +78,11 @@ class LidarStep:
             if ev[i] is not None:
                 ev[i].record(self.stream)
 
+        nfb = ctypes.c_int64(0)
+        total = ctypes.c_int64(0)
+        idx_p = ctypes.c_void_p(self.idx_d.data_ptr())
+        cnt_p = ctypes.c_void_p(self.cnt_d.data_ptr())
+        nrm_p = ctypes.c_void_p(self.nrm_d.data_ptr())
         with torch.cuda.stream(self.stream):
             mark(0)
             tree = capi.Tree.build(dev_ptr=self.pts_d.data_ptr(), n=self.n, dim=3,
@@ -79,12 +90,15 @@ class LidarStep:
             try:
                 h = ctypes.c_void_p(tree.handle)
                 mark(1)
-                capi.check(L.pptk_knn(h, ctypes.byref(self._qall), self.k, float("inf"),
-                                      ctypes.c_void_p(self.idx_d.data_ptr()),
-                                      ctypes.c_void_p(self.cnt_d.data_ptr()), None,
-                                      capi.MEM_DEVICE, sh))
+                if self.fused:
+                    capi.check(L.pptk_knn_normals(h, self.k, float("inf"), self.nk, float("inf"),
+                                                  1, self.precision, idx_p, cnt_p, None, nrm_p,
+                                                  None, None, ctypes.byref(nfb), capi.MEM_DEVICE,
+                                                  sh))
+                else:
+                    capi.check(L.pptk_knn(h, ctypes.byref(self._qall), self.k, float("inf"),
+                                          idx_p, cnt_p, None, capi.MEM_DEVICE, sh))
                 mark(2)
-                total = ctypes.c_int64(0)
                 if self.m:
                     capi.check(L.pptk_rnear_count(h, ctypes.byref(self._qpt), self.dmax,
                                                   ctypes.c_void_p(self.off_d.data_ptr()),
@@ -99,11 +113,11 @@ class LidarStep:
                                                  ctypes.c_void_p(self.ridx_d.data_ptr()),
                                                  capi.MEM_DEVICE, sh))
                 mark(4)
-                nfb = ctypes.c_int64(0)
-                capi.check(L.pptk_normals(h, self.nk, float("inf"), None, 0, 1, self.precision,
-                                          ctypes.c_void_p(self.nrm_d.data_ptr()), None, None,
-                                          ctypes.byref(nfb), capi.MEM_DEVICE, sh))
-                mark(5)
+                if not self.fused:
+                    capi.check(L.pptk_normals(h, self.nk, float("inf"), None, 0, 1,
+                                              self.precision, nrm_p, None, None,
+                                              ctypes.byref(nfb), capi.MEM_DEVICE, sh))
+                    mark(5)
                 if self.leaf_size is None:
                     self.leaf_size = tree.info.leaf_size
             finally:

@@ -12,7 +12,9 @@ pointer, every output left in device memory).
   neighbourhoods (oracle kNN including the point + LAPACK eigh) on 200,000
   sampled rows; tolerance 0.5 degrees. The distribution is written to
   $PPTK_REPORT_DIR/c1_full_parity.json when that directory exists.
-* Running the step again gives bit-identical outputs.
+* Running the step again gives bit-identical outputs, and the step with
+  the separate kNN and normals calls (LidarStep(fused=False)) gives
+  bit-identical outputs to the default one-search pptk_knn_normals call.
 """
 import json
 import os
@@ -57,6 +59,14 @@ def test_c1_full_size_through_the_timed_path(gpu):
     for a, b in zip((idx, cnt, off, lst, nrm), again):
         assert np.array_equal(a, b, equal_nan=True), "second run differs"
     del again
+    sep = pipeline.LidarStep(step.pts_d, step.q_d, k=K, dmax=DMAX, precision=capi.PREC_MIXED,
+                             fused=False)
+    sep.run()
+    for name, a, b in zip(("idx", "cnt", "off", "lists", "normals"), (idx, cnt, off, lst, nrm),
+                          sep.host_outputs()):
+        assert np.array_equal(a, b, equal_nan=True), "separate calls differ in " + name
+    assert sep.num_fp64 == step.num_fp64
+    del sep
     _say("GPU step done, %d r-near neighbours (%.1fs)" % (step.rtotal, time.time() - t0))
 
     ot = O.OracleTree(pts)
@@ -87,6 +97,7 @@ def test_c1_full_size_through_the_timed_path(gpu):
               "angle_deg_eigengap_lt_1e-3": _q(ang[gap < 1e-3]) if np.any(gap < 1e-3) else None,
               "fp64_fallback_fraction": step.num_fp64 / N,
               "knn_rows_identical": N, "rnear_rows_identical": M,
+              "fused_knn_normals_equal_separate_calls": True,
               "rnear_neighbours": int(woff[-1])}
     if os.path.isdir(REPORT_DIR):
         with open(os.path.join(REPORT_DIR, "c1_full_parity.json"), "w") as f:

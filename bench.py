@@ -344,12 +344,14 @@ def bench_lidar(args):
 
     log("bench: warmup %d, timing %d steps" % (args.warmup, args.steps))
     ms_step, ph = timed(pipe, args.warmup, args.steps, True)
-    # the same step with the two separate calls (two searches per point)
-    log("bench: the step with separate kNN and normals calls")
-    sep = pipeline.LidarStep(pts_d, q_d, k=k, dmax=dmax, leaf=args.leaf, precision=prec,
-                             fused=False)
-    sep_ms, sep_ph = timed(sep, 2, min(args.steps, 5), False)
-    del sep
+    sep_ms, sep_ph = None, None
+    if not args.only_step:
+        # the same step with the two separate calls (two searches per point)
+        log("bench: the step with separate kNN and normals calls")
+        sep = pipeline.LidarStep(pts_d, q_d, k=k, dmax=dmax, leaf=args.leaf, precision=prec,
+                                 fused=False)
+        sep_ms, sep_ph = timed(sep, 2, min(args.steps, 5), False)
+        del sep
 
     ab = algorithmic_bytes(n, k, m, pipe.rtotal)
     dom = max((p for p in PHASE_KERNEL if p in ph), key=lambda p: ph[p])
@@ -379,7 +381,7 @@ def bench_lidar(args):
         },
         "phases_ms": ph,
         "knn_normals_points_per_s": n / (ph["knn_normals"] * 1e-3),
-        "separate_calls": {
+        "separate_calls": None if sep_ph is None else {
             "what": "the same step with pptk_knn then pptk_normals (a search each) in place "
                     "of pptk_knn_normals; outputs bit-identical",
             "ms_per_step": sep_ms, "phases_ms": sep_ph,
@@ -408,6 +410,8 @@ def bench_lidar(args):
     torch.cuda.synchronize()
     torch.cuda.empty_cache()
     capi.lib().pptk_trim()
+    if args.only_step:
+        return out
     log("bench: drop-in API timing")
     out["api"] = api_times(pts, k)
     if not args.no_cpu:
@@ -752,6 +756,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="lidar: skip the batched and served summaries")
+    ap.add_argument("--only-step", action="store_true",
+                    help="lidar: only the timed step (no separate-call step, API timing, CPU "
+                         "baseline or extras; profiling runs)")
     ap.add_argument("--no-plain", action="store_true",
                     help="batched: skip the plain-path comparison (profiling runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -823,7 +830,7 @@ def main():
         print(json.dumps(bench_served(args)), flush=True)
         return
     out = bench_lidar(args)
-    if not args.no_extras:
+    if not (args.no_extras or args.only_step):
         plain10 = out["plain_10m_points_per_s"]
         ex = argparse.Namespace(**vars(args))
         ex.k, ex.leaf, ex.qblock, ex.no_plain = 16, 192, 64, False

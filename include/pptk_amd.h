@@ -271,7 +271,7 @@ typedef struct pptk_batch pptk_batch;
 
 typedef struct {
   int k;               /* self-query kNN k, 0..32 (0 = no kNN output) */
-  int normals_k;       /* normals neighbourhood size incl. the point, 0..33
+  int normals_k;       /* normals neighbourhood size incl. the point, 0..32
                           (0 = no normals); dim must be 3 */
   double dmax;         /* distance bound of both (INFINITY = none) */
   int max_leaf_size;   /* bucket size 1..256, 0 = tuned default */

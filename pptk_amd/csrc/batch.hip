@@ -661,8 +661,8 @@ pptk_status api_batch_create(const int64_t* sizes, int64_t C, int dim, pptk_dtyp
       fail(PPTK_E_TYPE, "batched clouds take float32 or float64 points");
     if (!prm) fail(PPTK_E_VALUE, "null batch parameters");
     if (prm->k < 0 || prm->k > 32) fail(PPTK_E_VALUE, "batch k must be in [0, 32]");
-    if (prm->normals_k < 0 || prm->normals_k > 33)
-      fail(PPTK_E_VALUE, "batch normals_k must be in [0, 33]");
+    if (prm->normals_k < 0 || prm->normals_k > 32)
+      fail(PPTK_E_VALUE, "batch normals_k must be in [0, 32]");
     if (prm->normals_k > 0 && dim != 3) fail(PPTK_E_VALUE, "normals need 3-D points");
     if (!(prm->dmax >= 0.0)) fail(PPTK_E_VALUE, "dmax must be non-negative");
     if (prm->precision < PPTK_PREC_AUTO || prm->precision > PPTK_PREC_FP64)

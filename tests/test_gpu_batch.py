@@ -184,7 +184,8 @@ def test_batch_graph_replay_and_new_input(gpu):
 
 
 @pytest.mark.parametrize("k,nk,dmax", [(8, 16, np.inf), (32, 16, np.inf), (16, 0, np.inf),
-                                       (0, 16, np.inf), (16, 16, 0.3), (4, 5, np.inf)])
+                                       (0, 16, np.inf), (16, 16, 0.3), (4, 5, np.inf),
+                                       (5, 32, np.inf)])
 def test_batch_k_variants(gpu, k, nk, dmax):
     rng = np.random.default_rng(14 + k + nk)
     clouds = _edge_clouds(rng)
@@ -232,6 +233,8 @@ def test_batch_argument_errors(gpu):
     from pptk_amd import capi
     with pytest.raises(ValueError):
         capi.Batch([10, 20], k=33)
+    with pytest.raises(ValueError):
+        capi.Batch([10, 20], k=16, normals_k=33)
     with pytest.raises(ValueError):
         capi.Batch([10, 20], dim=2, normals_k=16)
     with pytest.raises(ValueError):

@@ -38,6 +38,7 @@ namespace pk {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kMortonExtraBits = 7;
 constexpr int kPer = 16;                 // rows of kBlock points per tile
 constexpr int kTile = kBlock * kPer;     // 4096
 
@@ -222,7 +223,8 @@ __global__ __launch_bounds__(kBlock) void morton_kernel(const T* __restrict__ pt
                                                         const uint32_t* __restrict__ blk_off,
                                                         const BuildInfo<D>* __restrict__ info_p,
                                                         uint64_t* __restrict__ keys,
-                                                        uint32_t* __restrict__ vals) {
+                                                        uint32_t* __restrict__ vals,
+                                                        int kshift) {
   constexpr int kWaves = kBlock / 64;
   static_assert(kPer * kWaves == 64, "one wave scans the (row, wave) counts");
   __shared__ uint32_t s_off[kPer * kWaves];
@@ -266,7 +268,7 @@ __global__ __launch_bounds__(kBlock) void morton_kernel(const T* __restrict__ pt
     D c[DIM];
 #pragma unroll
     for (int d = 0; d < DIM; ++d) c[d] = (D)pts[i * DIM + d];
-    keys[pos] = morton_code<D, DIM>(c, info);
+    keys[pos] = morton_code<D, DIM>(c, info) >> kshift;
     vals[pos] = (uint32_t)i;
   }
 }
@@ -361,12 +363,19 @@ void build_impl(pptk_tree* t, const T* dpts, int64_t n, hipStream_t s) {
     // K1: compaction + Morton codes
     keys.reset((size_t)nv * 8);
     vals.reset((size_t)nv * 4);
+    // Key bits per axis: enough for ~2^kMortonExtraBits more cells per axis
+    // than the points need on a surface (ceil(log2 nv / DIM) + extra), at most
+    // the full code; the low bits are dropped, so fewer radix passes.
+    int lg = 0;
+    while ((int64_t(1) << lg) < nv) ++lg;
+    const int kbits = std::min(MortonBits<DIM>::value, (lg + DIM - 1) / DIM + kMortonExtraBits);
+    const int kshift = DIM * (MortonBits<DIM>::value - kbits);
     morton_kernel<T, D, DIM><<<(unsigned)nblk, kBlock, 0, s>>>(
         dpts, n, blk_off.as<uint32_t>(), info_d.as<BuildInfo<D>>(), keys.as<uint64_t>(),
-        vals.as<uint32_t>());
+        vals.as<uint32_t>(), kshift);
     PK_LAUNCH_CHECK();
     // K2: sort
-    radix_sort_pairs(keys.as<uint64_t>(), vals.as<uint32_t>(), nv, DIM * MortonBits<DIM>::value, s);
+    radix_sort_pairs(keys.as<uint64_t>(), vals.as<uint32_t>(), nv, DIM * kbits, s);
     // K3: gather in tree order
     gather_kernel<T, D, DIM><<<grid_for(nv, kBlock), kBlock, 0, s>>>(
         dpts, vals.as<uint32_t>(), nv, t->pts.as<PointRec<D, DIM>>(), t->orig.as<int32_t>(),

@@ -21,7 +21,8 @@
 //
 // Device-driven level loop: the top-down split and the bottom-up box pass run
 // as a fixed number of grid-stride launches (one per possible level; the
-// depth is bounded by the Morton bits per cloud plus 31 median halvings).
+// depth is bounded by the Morton bits per cloud plus the halvings that bring
+// the largest cloud's equal-key runs to the bucket size).
 // Each split launch reads its node range from device memory and its last
 // block to finish publishes the next level's range, so the host never reads
 // the node counter.
@@ -700,7 +701,14 @@ pptk_status api_batch_create(const int64_t* sizes, int64_t C, int dim, pptk_dtyp
     b->mb = std::min({mbmax, (63 - cbits) / dim, need});
     b->shift = dim * b->mb;
     b->end_bit = b->shift + cbits;
-    b->nlev = b->shift + 33;
+    // Levels: along any root-to-leaf path a split either lowers the highest
+    // differing key bit (at most `shift` such splits inside one cloud) or
+    // halves a node whose keys are all equal, after which only such halvings
+    // follow; ceil(log2(maxn / leaf)) of them bring any run to <= leaf. So
+    // the depth is at most 1 + shift + that, and no launch goes past it.
+    int halvings = 0;
+    while (((int64_t)b->leaf << halvings) < maxn) ++halvings;
+    b->nlev = b->shift + halvings + 2;
     b->k = prm->k;
     b->nk = prm->normals_k;
     b->kk = std::max(b->k, b->nk);

@@ -253,6 +253,27 @@ __device__ __forceinline__ D box_max_dist2(const BoxRec<D, DIM>& b, const D* q) 
   return s;
 }
 
+// Workgroups of a launch are dispatched to the chip's 8 XCDs in turn
+// (workgroup i to XCD i mod 8), and each XCD has its own L2. xcd_block maps
+// the dispatched workgroup b of a grid of g to the index of the work it
+// serves so that, in every span of 8 * run consecutive workgroups, each XCD
+// takes a run of `run` consecutive work indices: neighbouring work (queries
+// that are neighbours on the Morton curve, reading the same leaves) then
+// shares one L2. Launch order is kept from one span to the next; the last,
+// partial span keeps the identity.
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t g, int run) {
+  const int64_t span = 8 * (int64_t)run;
+  const int64_t s = b / span;
+  if (run <= 1 || (s + 1) * span > g) return b;
+  const int64_t r = b - s * span;
+  return s * span + (r & 7) * run + (r >> 3);
+}
+
+// Run length of the self-query search kernels and the r-near count (lidar
+// scene, 10M points: kNN + normals 4.50 -> 4.40 ms, count 1.65 -> 1.60 ms
+// at 16; 2 to 64 measure alike).
+constexpr int kXcdRun = 16;
+
 template <typename D> inline D qnan_host() { return std::numeric_limits<D>::quiet_NaN(); }
 
 template <typename D> __device__ __forceinline__ bool finite_val(D x) {

@@ -113,12 +113,13 @@ __global__ __launch_bounds__(kBlock, K <= 16 ? 8 : 1) void normals_coop_kernel(T
                                                               float* __restrict__ evecs,
                                                               float* __restrict__ evals,
                                                               int32_t* __restrict__ sizes,
-                                                              unsigned long long* __restrict__ nfb) {
+                                                              unsigned long long* __restrict__ nfb,
+                                                              int xrun) {
   __shared__ uint64_t lds[kBlock / 64 * coop_wave_bytes<K, 3, K>() / 8];
   // the wave's first position and LDS area are scalars; a lane's position
   // is recomputed after the search rather than held through it
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int64_t t0 = (int64_t)blockIdx.x * kBlock + wave * 64;
+  const int64_t t0 = xcd_block(blockIdx.x, gridDim.x, xrun) * kBlock + wave * 64;
   int64_t t = t0 + __lane_id();
   float q[3] = {0.0f, 0.0f, 0.0f};
   bool active = false;
@@ -183,12 +184,12 @@ __global__ __launch_bounds__(kBlock, K <= 16 ? 8 : 1) void knn_normals_coop_kern
     TreeView<float, 3> tv, int64_t m, int k, int kk, int nk, float r2s, float r2k, float r2n,
     int32_t* __restrict__ out_idx, int32_t* __restrict__ out_cnt, float* __restrict__ out_dist,
     float* __restrict__ evecs, float* __restrict__ evals, int32_t* __restrict__ sizes,
-    unsigned long long* __restrict__ nfb) {
+    unsigned long long* __restrict__ nfb, int xrun) {
   __shared__ uint64_t lds[kBlock / 64 * coop_wave_bytes<K, 3, K>() / 8];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   // tree positions fit int32; the lane's own is rebuilt from the wave's
   // first after the search instead of being held through it
-  const int32_t t0 = (int32_t)blockIdx.x * kBlock + wave * 64;
+  const int32_t t0 = (int32_t)xcd_block(blockIdx.x, gridDim.x, xrun) * kBlock + wave * 64;
   const int32_t t = t0 + (int32_t)__lane_id();
   float q[3] = {0.0f, 0.0f, 0.0f};
   int32_t self = -1;
@@ -423,7 +424,8 @@ void normals_launch(const pptk_tree* t, int k, D r2, const int32_t* drows, int64
       if constexpr (std::is_same<D, float>::value) {
         if (!drows) {
           normals_coop_kernel<K, PREC, NE><<<grid_of(m), kBlock, 0, s>>>(tv, m, k, r2, evecs,
-                                                                         evals, sizes, nfb);
+                                                                         evals, sizes, nfb,
+                                                                         kXcdRun);
           PK_LAUNCH_CHECK();
           return;
         }
@@ -562,7 +564,8 @@ void knn_normals_run(const pptk_tree* t, int k, double dmax, int nk, double r, i
       constexpr int NE = decltype(ec)::value;
       knn_normals_coop_kernel<K, P, NE><<<grid_of(m), kBlock, 0, s>>>(
           tv, m, k, kk, nk, r2s, r2k, r2n, (int32_t*)oi.dptr, (int32_t*)oc.dptr,
-          (float*)od.dptr, evecs, evals, sizes, nfb.as<unsigned long long>());
+          (float*)od.dptr, evecs, evals, sizes, nfb.as<unsigned long long>(),
+          kXcdRun);
       PK_LAUNCH_CHECK();
     };
     auto by_k = [&](auto pc, auto ec) {

@@ -8,6 +8,7 @@
 // Reference entry points: KdTree::KNearestNeighbors[Self] and
 // RNearNeighbors[Self] (pptk/kdtree/src/kdtree-impl.h:873-1069).
 #include <algorithm>
+#include <cstdlib>
 #include <type_traits>
 #include <vector>
 
@@ -69,9 +70,10 @@ __global__ __launch_bounds__(kBlock, K <= 16 && DIM <= 3 ? 8 : 1) void knn_coop_
                                                           int64_t nrows, int k, float r2,
                                                           int32_t* __restrict__ out_idx,
                                                           int32_t* __restrict__ out_cnt,
-                                                          float* __restrict__ out_dist) {
+                                                          float* __restrict__ out_dist,
+                                                          int xrun) {
   __shared__ uint64_t lds[kBlock / 64 * coop_wave_bytes<K, DIM, 0>() / 8];
-  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t t = xcd_block(blockIdx.x, gridDim.x, xrun) * kBlock + threadIdx.x;
   float q[DIM];
 #pragma unroll
   for (int d = 0; d < DIM; ++d) q[d] = 0.0f;
@@ -339,6 +341,30 @@ __global__ __launch_bounds__(kBlock) void rnear_kernel(TreeView<D, DIM> tv, int 
   }
 }
 
+// Event lists of an r-near count pass, replayed by the fill pass of the same
+// queries so that the fill does not walk the tree again. An event is a node
+// at which some of a count wave's lanes take work: lanes whose ball holds
+// the node's box (low 32 bits of the mask: the whole range [a, b) is theirs)
+// and lanes whose ball straddles a leaf (high 32 bits: the leaf's points are
+// tested one by one). A count wave (pk <= 32 queries) writes its events in
+// its traversal order into chunks of 64 taken from a shared pool by one
+// atomic add each; its header holds the first chunk and the event count, or
+// -1 when the pool ran out (its fill waves then walk the tree). A fill wave
+// serves a contiguous subset of a count wave's lanes; it replays the events
+// in the recorded order, so every list is written in the same (tree) order
+// as by a walk.
+constexpr int kEventChunk = 64;
+struct RnearEventView {
+  int32_t* node = nullptr;        // [nchunks * 64]: event node
+  uint64_t* mask = nullptr;       // [nchunks * 64]: inside lanes | straddling lanes << 32
+  int32_t* link = nullptr;        // [nchunks]: next chunk of the same wave
+  int32_t* hdr = nullptr;         // [2 * count waves]: first chunk, events (-1: not kept)
+  uint32_t* head = nullptr;       // chunks taken
+  int32_t nchunks = 0;
+  int32_t pk = 0;                 // queries per count wave
+  const int32_t* same = nullptr;  // fill: nonzero if its queries are the count's
+};
+
 // r-near on a single-root tree, one traversal per wave (count: FILL = false,
 // fill: FILL = true). A wave takes pk <= 64 consecutive queries (lanes pk..63
 // idle); consecutive queries are neighbours on the Morton curve, so their
@@ -355,6 +381,9 @@ __global__ __launch_bounds__(kBlock) void rnear_kernel(TreeView<D, DIM> tv, int 
 //     and tested against each straddling lane's query in turn, the hits
 //     ranked with a ballot; at an inner node the wave descends.
 // Each lane's output is in tree order, exactly as rnear_kernel's.
+// The count records its events when ev.node is set (pk <= 32); the fill
+// replays them when ev.hdr is set, *ev.same is nonzero and its count wave
+// kept its list, and walks the tree otherwise.
 // EXCL: self queries (tree points): the query's own position is skipped.
 // Lane values needed by the whole wave (query, own position, write pointer
 // and end) live in LDS and are fetched with one broadcast read; the hit
@@ -370,9 +399,10 @@ __global__ __launch_bounds__(kBlock) void rnear_packet_kernel(TreeView<D, DIM> t
                                                               int32_t* __restrict__ counts,
                                                               const int64_t* __restrict__ offsets,
                                                               int64_t total,
-                                                              int32_t* __restrict__ out) {
+                                                              int32_t* __restrict__ out,
+                                                              RnearEventView ev, int xrun) {
   const int lane = threadIdx.x & 63;
-  int64_t w = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  int64_t w = xcd_block(blockIdx.x, gridDim.x, xrun) * (kBlock / 64) + (threadIdx.x >> 6);
   if (wave_order && w < nw) w = wave_order[w];
   const int64_t t = w * pk + lane;
   D q[DIM];
@@ -412,104 +442,196 @@ __global__ __launch_bounds__(kBlock) void rnear_packet_kernel(TreeView<D, DIM> t
       ww[2 * lane + 1] = e;
     }
   }
-  int32_t cnt = 0;                  // count: outputs so far for this lane's query
-  int32_t done_end = INT32_MIN;     // nodes starting before this are taken
+  int32_t cnt = 0;  // count: outputs so far for this lane's query
   // wave masks are ballots of bare comparisons combined as scalars (a ballot
   // of a combined per-lane predicate costs two more vector instructions)
   const uint64_t actm = __ballot(active);
-  int32_t node = actm ? 0 : -1;
-  while (node >= 0) {
-    node = __builtin_amdgcn_readfirstlane(node);
-    const NodeRec nr = tv.node[node];
-    const BoxRec<D, DIM> bx = tv.box[node];
-    const bool c_new = nr.a >= done_end;
-    const bool c_min = box_min_dist2<D, DIM>(bx, q) < r2;
-    const bool hit = active & c_new & c_min;
-    const uint64_t hitm = __ballot(c_new) & __ballot(c_min) & actm;
-    bool descend = false;
-    if (hitm) {
-      const bool c_max = box_max_dist2<D, DIM>(bx, q) < r2;
-      const bool inside = hit & c_max;
-      const uint64_t cmaxm = __ballot(c_max);
-      const uint64_t insm = hitm & cmaxm;
-      const uint64_t strm = hitm & ~cmaxm;
-      if (inside) done_end = nr.b;
-      if constexpr (FILL) {
-        if (insm) {
-          // copy the range once per chunk, into every inside lane's list
-          for (int32_t s0 = nr.a; s0 < nr.b; s0 += 64) {
-            const int32_t p = s0 + lane;
-            const bool valid = p < nr.b;
-            const int32_t og = valid ? tv.orig[p] : 0;
-            uint64_t lm = insm;
-            while (lm) {
-              const int L = __builtin_ctzll(lm);
-              lm &= lm - 1ull;
-              const int64_t wL = ww[2 * L];
-              const int64_t eL = ww[2 * L + 1];
-              int64_t o = wL + (p - nr.a);
-              bool w = valid;
-              if constexpr (EXCL) {
-                const int32_t exL = wq[L].orig;
-                // the own position is skipped: later points move down one slot
-                o -= ((exL >= nr.a) & (p > exL)) ? 1 : 0;
-                w &= p != exL;
-              }
-              if (w & (o < eL)) out[o] = og;
-            }
-          }
+
+  // Output size of a node for this lane (its own position excluded).
+  auto range_size = [&](const NodeRec& nr) {
+    int32_t size = nr.b - nr.a;
+    if constexpr (EXCL) size -= ((excl >= nr.a) & (excl < nr.b)) ? 1 : 0;
+    return size;
+  };
+  // Fill: copies the range of a node into the list of every lane of insm,
+  // once per 64-point chunk.
+  auto copy_range = [&](const NodeRec& nr, uint64_t insm) {
+    for (int32_t s0 = nr.a; s0 < nr.b; s0 += 64) {
+      const int32_t p = s0 + lane;
+      const bool valid = p < nr.b;
+      const int32_t og = valid ? tv.orig[p] : 0;
+      uint64_t lm = insm;
+      while (lm) {
+        const int L = __builtin_ctzll(lm);
+        lm &= lm - 1ull;
+        const int64_t wL = ww[2 * L];
+        const int64_t eL = ww[2 * L + 1];
+        int64_t o = wL + (p - nr.a);
+        bool wr = valid;
+        if constexpr (EXCL) {
+          const int32_t exL = wq[L].orig;
+          // the own position is skipped: later points move down one slot
+          o -= ((exL >= nr.a) & (p > exL)) ? 1 : 0;
+          wr &= p != exL;
         }
-        if (inside) {
-          int32_t size = nr.b - nr.a;
-          if constexpr (EXCL) size -= ((excl >= nr.a) & (excl < nr.b)) ? 1 : 0;
-          ww[2 * lane] += size;
-        }
-      } else {
-        if (inside) {
-          int32_t size = nr.b - nr.a;
-          if constexpr (EXCL) size -= ((excl >= nr.a) & (excl < nr.b)) ? 1 : 0;
-          cnt += size;
-        }
+        if (wr & (o < eL)) out[o] = og;
       }
-      if (strm) {
-        if (nr.child < 0) {
-          for (int32_t s0 = nr.a; s0 < nr.b; s0 += 64) {
-            const int32_t p = s0 + lane;
-            const bool valid = p < nr.b;
-            PointRec<D, DIM> rec{};
-            if (valid) rec = tv.pts[p];
-            // hit masks are ballots of the bare distance test, combined with
-            // the chunk's valid lanes (and the own position) as scalar masks
-            const uint64_t vmask = __ballot(valid);
-            uint64_t sm = strm;
-            while (sm) {
-              const int L = __builtin_ctzll(sm);
-              sm &= sm - 1ull;
-              const PointRec<D, DIM> qL = wq[L];
-              const bool in = dist2<D, DIM>(qL.c, rec.c) < r2;
-              uint64_t hits = __ballot(in) & vmask;
-              int32_t exL = -1;
-              if constexpr (EXCL) {
-                exL = __builtin_amdgcn_readfirstlane(qL.orig);
-                const uint32_t dl = (uint32_t)(exL - s0);
-                hits &= dl < 64u ? ~(1ull << dl) : ~0ull;
-              }
-              if constexpr (FILL) {
-                const int64_t wL = ww[2 * L];
-                const int64_t o = wL + rank_below(hits);
-                if (in & valid & (p != exL) & (o < ww[2 * L + 1])) out[o] = rec.orig;
-                if (lane == 0) ww[2 * L] = wL + __popcll(hits);
-              } else {
-                cnt += lane == L ? (int32_t)__popcll(hits) : 0;
-              }
-            }
-          }
+    }
+  };
+  // Tests a leaf's points against each lane of strm: counts the hits, or
+  // writes them (fill).
+  auto scan_leaf = [&](const NodeRec& nr, uint64_t strm) {
+    for (int32_t s0 = nr.a; s0 < nr.b; s0 += 64) {
+      const int32_t p = s0 + lane;
+      const bool valid = p < nr.b;
+      PointRec<D, DIM> rec{};
+      if (valid) rec = tv.pts[p];
+      // hit masks are ballots of the bare distance test, combined with
+      // the chunk's valid lanes (and the own position) as scalar masks
+      const uint64_t vmask = __ballot(valid);
+      uint64_t sm = strm;
+      while (sm) {
+        const int L = __builtin_ctzll(sm);
+        sm &= sm - 1ull;
+        const PointRec<D, DIM> qL = wq[L];
+        const bool in = dist2<D, DIM>(qL.c, rec.c) < r2;
+        uint64_t hits = __ballot(in) & vmask;
+        int32_t exL = -1;
+        if constexpr (EXCL) {
+          exL = __builtin_amdgcn_readfirstlane(qL.orig);
+          const uint32_t dl = (uint32_t)(exL - s0);
+          hits &= dl < 64u ? ~(1ull << dl) : ~0ull;
+        }
+        if constexpr (FILL) {
+          const int64_t wL = ww[2 * L];
+          const int64_t o = wL + rank_below(hits);
+          if (in & valid & (p != exL) & (o < ww[2 * L + 1])) out[o] = rec.orig;
+          if (lane == 0) ww[2 * L] = wL + __popcll(hits);
         } else {
-          descend = true;
+          cnt += lane == L ? (int32_t)__popcll(hits) : 0;
         }
       }
     }
-    node = descend ? nr.child : next_in_order<D, DIM>(tv, node);
+  };
+
+  bool walk = true;
+  if constexpr (FILL) {
+    if (ev.hdr && actm) {
+      // this wave's lanes are lanes h0 .. h0 + pk - 1 of count wave cw
+      const int64_t t0 = w * pk;
+      const int64_t cw = t0 / ev.pk;
+      const int h0 = (int)(t0 - cw * ev.pk);
+      const int32_t first = ev.hdr[2 * cw];
+      const int32_t nev = ev.hdr[2 * cw + 1];
+      if (*ev.same != 0 && nev >= 0) {
+        walk = false;
+        const uint32_t lm = pk >= 32 ? 0xffffffffu : ((1u << pk) - 1u);
+        int32_t chunk = first;
+        for (int32_t e0 = 0; e0 < nev; e0 += kEventChunk) {
+          const bool has = lane < nev - e0;
+          const int64_t slot = (int64_t)chunk * kEventChunk + lane;
+          const int32_t en = has ? ev.node[slot] : 0;
+          const uint64_t em = has ? ev.mask[slot] : 0ull;
+          const uint32_t mi = ((uint32_t)em >> h0) & lm;
+          const uint32_t ms = ((uint32_t)(em >> 32) >> h0) & lm;
+          uint64_t todo = __ballot((mi | ms) != 0u);
+          const int32_t next = e0 + kEventChunk < nev ? ev.link[chunk] : -1;
+          while (todo) {
+            const int j = __builtin_ctzll(todo);
+            todo &= todo - 1ull;
+            const int32_t node = __builtin_amdgcn_readlane(en, j);
+            const uint64_t insm = (uint32_t)__builtin_amdgcn_readlane((int)mi, j);
+            const uint64_t strm = (uint32_t)__builtin_amdgcn_readlane((int)ms, j);
+            const NodeRec nr = tv.node[node];
+            if (insm) {
+              copy_range(nr, insm);
+              if ((insm >> lane) & 1ull) ww[2 * lane] += range_size(nr);
+            }
+            if (strm) scan_leaf(nr, strm);
+          }
+          chunk = next;
+        }
+      }
+    }
+  }
+  if (walk) {
+    // the count's event recording (uniform state; lane j of en/em holds
+    // event j of the batch not yet written)
+    bool rec = false;
+    if constexpr (!FILL) rec = ev.node != nullptr && w < nw;
+    int32_t en = 0;
+    uint64_t em = 0;
+    int32_t e_pos = 0, e_first = -1, e_chunk = -1, e_count = 0;
+    auto flush = [&]() {
+      int32_t c = 0;
+      if (lane == 0) c = (int32_t)atomicAdd(ev.head, 1u);
+      c = __builtin_amdgcn_readlane(c, 0);
+      if (c < 0 || c >= ev.nchunks) {
+        rec = false;  // pool exhausted: the fill walks for this wave
+        return;
+      }
+      const int64_t slot = (int64_t)c * kEventChunk + lane;
+      if (lane < e_pos) {
+        ev.node[slot] = en;
+        ev.mask[slot] = em;
+      }
+      if (e_chunk >= 0) {
+        if (lane == 0) ev.link[e_chunk] = c;
+      } else {
+        e_first = c;
+      }
+      e_chunk = c;
+      e_count += e_pos;
+      e_pos = 0;
+    };
+    int32_t done_end = INT32_MIN;  // nodes starting before this are taken
+    int32_t node = actm ? 0 : -1;
+    while (node >= 0) {
+      node = __builtin_amdgcn_readfirstlane(node);
+      const NodeRec nr = tv.node[node];
+      const BoxRec<D, DIM> bx = tv.box[node];
+      const bool c_new = nr.a >= done_end;
+      const bool c_min = box_min_dist2<D, DIM>(bx, q) < r2;
+      const bool hit = active & c_new & c_min;
+      const uint64_t hitm = __ballot(c_new) & __ballot(c_min) & actm;
+      bool descend = false;
+      if (hitm) {
+        const bool c_max = box_max_dist2<D, DIM>(bx, q) < r2;
+        const bool inside = hit & c_max;
+        const uint64_t cmaxm = __ballot(c_max);
+        const uint64_t insm = hitm & cmaxm;
+        const uint64_t strm = hitm & ~cmaxm;
+        if (inside) done_end = nr.b;
+        if constexpr (FILL) {
+          if (insm) copy_range(nr, insm);
+          if (inside) ww[2 * lane] += range_size(nr);
+        } else {
+          if (inside) cnt += range_size(nr);
+          if (rec) {
+            const uint64_t emask = insm | (nr.child < 0 ? strm << 32 : 0ull);
+            if (emask) {
+              en = lane == e_pos ? node : en;
+              em = lane == e_pos ? emask : em;
+              if (++e_pos == kEventChunk) flush();
+            }
+          }
+        }
+        if (strm) {
+          if (nr.child < 0) scan_leaf(nr, strm);
+          else descend = true;
+        }
+      }
+      node = descend ? nr.child : next_in_order<D, DIM>(tv, node);
+    }
+    if constexpr (!FILL) {
+      if (ev.node != nullptr && w < nw) {
+        if (rec && e_pos > 0) flush();
+        if (lane == 0) {
+          ev.hdr[2 * w] = e_first;
+          ev.hdr[2 * w + 1] = rec ? e_count : -1;
+        }
+      }
+    }
   }
   if constexpr (!FILL) {
     if (has_row) counts[row] = ok ? cnt : 0;
@@ -624,6 +746,7 @@ void fill_async(T* p, int64_t n, T v, hipStream_t s) {
   PK_LAUNCH_CHECK();
 }
 template void fill_async<int32_t>(int32_t*, int64_t, int32_t, hipStream_t);
+template void fill_async<uint32_t>(uint32_t*, int64_t, uint32_t, hipStream_t);
 template void fill_async<float>(float*, int64_t, float, hipStream_t);
 template void fill_async<double>(double*, int64_t, double, hipStream_t);
 
@@ -865,7 +988,8 @@ void knn_run(const pptk_tree* t, const pptk_queries* qs, int k, double dmax, int
         if constexpr (std::is_same<D, float>::value) {
           if (pq.mode == kQueryAll) {
             knn_coop_kernel<DIM, K><<<grid_of(m), kBlock, 0, s>>>(
-                tv, m, nrows, k, r2, (int32_t*)oi.dptr, (int32_t*)oc.dptr, (float*)od.dptr);
+                tv, m, nrows, k, r2, (int32_t*)oi.dptr, (int32_t*)oc.dptr, (float*)od.dptr,
+                kXcdRun);
             PK_LAUNCH_CHECK();
             return;
           }
@@ -913,6 +1037,32 @@ void knn_run(const pptk_tree* t, const pptk_queries* qs, int k, double dmax, int
   PK_HIP(hipStreamSynchronize(s));
 }
 
+// Chunks of 64 events per count wave in a fresh pool, and the pool's cap
+// (PPTK_RNEAR_EVENT_CHUNKS overrides the cap; 0 keeps no lists).
+constexpr double kEventChunksPerWave = 4.0;
+constexpr int64_t kEventPoolBytes = (int64_t)1 << 30;
+constexpr size_t kEventChunkBytes = (size_t)kEventChunk * 12 + 4;
+
+int64_t event_pool_cap() {
+  const char* e = getenv("PPTK_RNEAR_EVENT_CHUNKS");
+  if (e && *e) {
+    char* end = nullptr;
+    const long long v = strtoll(e, &end, 10);
+    if (end && *end == '\0' && v >= 0) return (int64_t)v;
+  }
+  return kEventPoolBytes / (int64_t)kEventChunkBytes;
+}
+
+__global__ __launch_bounds__(kBlock) void words_equal_kernel(const uint32_t* __restrict__ a,
+                                                             const uint32_t* __restrict__ b,
+                                                             int64_t n, int32_t* __restrict__ same) {
+  bool diff = false;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock)
+    diff |= a[i] != b[i];
+  if (diff) *same = 0;
+}
+
 template <typename D, int DIM>
 void rnear_count_run(const pptk_tree* t, const pptk_queries* qs, double dmax, int64_t* out_offsets,
                      int64_t* out_total, pptk_mem out_mem, hipStream_t s) {
@@ -926,22 +1076,69 @@ void rnear_count_run(const pptk_tree* t, const pptk_queries* qs, double dmax, in
   DevBuf counts((size_t)std::max<int64_t>(nrows, 1) * 4);
   if (nrows > 0) PK_HIP(hipMemsetAsync(counts.get(), 0, (size_t)nrows * 4, s));
   const TreeView<D, DIM> tv = tree_view<D, DIM>(t);
+  std::shared_ptr<RnearEvents> ent;
+  uint32_t used = 0;
   if (pq.m > 0) {
     const int pk = kRnearCountWidth;
-    const unsigned grid = (unsigned)std::max<int64_t>(1, ceil_div(ceil_div(pq.m, pk), kBlock / 64));
+    const int64_t nw = ceil_div(pq.m, pk);
+    const unsigned grid = (unsigned)std::max<int64_t>(1, ceil_div(nw, kBlock / 64));
+    RnearEventView ev;
+    if (tv.nroots == 1) {
+      // event lists for the fill of the same queries
+      double per_wave = kEventChunksPerWave;
+      {
+        std::lock_guard<std::mutex> lk(t->rnear_cache.mu);
+        per_wave = std::max(per_wave, t->rnear_cache.chunks_per_wave);
+      }
+      const int64_t nchunks = std::min<int64_t>(event_pool_cap(),
+                                                (int64_t)(per_wave * (double)nw) + 64);
+      if (nchunks > 0) {
+        ent = std::make_shared<RnearEvents>();
+        ent->src = pq.src;
+        ent->m = pq.m;
+        ent->mode = pq.mode;
+        ent->r = (double)r;
+        ent->pk = pk;
+        ent->nw = nw;
+        ent->nchunks = nchunks;
+        ent->order = pq.order ? pq.order_buf : nullptr;
+        if (pq.mode == kQueryPoints) ent->qbytes = (size_t)pq.m * DIM * sizeof(D);
+        else if (pq.mode == kQueryList) ent->qbytes = (size_t)pq.m * 4;
+        if (ent->qbytes) {
+          ent->queries.reset(ent->qbytes);
+          PK_HIP(hipMemcpyAsync(ent->queries.get(),
+                                pq.mode == kQueryPoints ? pq.dpts : (const void*)pq.dlist,
+                                ent->qbytes, hipMemcpyDeviceToDevice, s));
+        }
+        ent->node.reset((size_t)nchunks * kEventChunk * 4);
+        ent->mask.reset((size_t)nchunks * kEventChunk * 8);
+        ent->link.reset((size_t)nchunks * 4);
+        ent->hdr.reset((size_t)nw * 8);
+        ent->head.reset(4);
+        fill_async<uint32_t>(ent->head.as<uint32_t>(), 1, 0u, s);
+        ev.node = ent->node.as<int32_t>();
+        ev.mask = ent->mask.as<uint64_t>();
+        ev.link = ent->link.as<int32_t>();
+        ev.hdr = ent->hdr.as<int32_t>();
+        ev.head = ent->head.as<uint32_t>();
+        ev.nchunks = (int32_t)nchunks;
+        ev.pk = pk;
+      }
+    }
     if (tv.nroots == 1 && pq.mode == kQueryPoints)
       rnear_packet_kernel<D, DIM, false, false><<<grid, kBlock, 0, s>>>(
-          tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.order, pq.m, nrows, r2, pk,
-          ceil_div(pq.m, pk), nullptr, counts.as<int32_t>(), nullptr, 0, nullptr);
+          tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.order, pq.m, nrows, r2, pk, nw, nullptr,
+          counts.as<int32_t>(), nullptr, 0, nullptr, ev, kXcdRun);
     else if (tv.nroots == 1)
       rnear_packet_kernel<D, DIM, false, true><<<grid, kBlock, 0, s>>>(
-          tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.order, pq.m, nrows, r2, pk,
-          ceil_div(pq.m, pk), nullptr, counts.as<int32_t>(), nullptr, 0, nullptr);
+          tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.order, pq.m, nrows, r2, pk, nw, nullptr,
+          counts.as<int32_t>(), nullptr, 0, nullptr, ev, kXcdRun);
     else
       rnear_kernel<D, DIM, false><<<grid_of(pq.m), kBlock, 0, s>>>(
           tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.order, pq.m, nrows, r2,
           counts.as<int32_t>(), nullptr, 0, nullptr);
     PK_LAUNCH_CHECK();
+    if (ent) PK_HIP(hipMemcpyAsync(&used, ent->head.get(), 4, hipMemcpyDeviceToHost, s));
   }
   int64_t* doff = (int64_t*)oo.dptr;
   DevBuf tmp;
@@ -955,6 +1152,14 @@ void rnear_count_run(const pptk_tree* t, const pptk_queries* qs, double dmax, in
   oo.finish(s);
   PK_HIP(hipStreamSynchronize(s));
   if (out_total) *out_total = total;
+  if (ent) {
+    ent->chunks_used = (int64_t)used;
+    // the next count on this tree starts with a pool this count would have
+    // filled to 4/5
+    std::lock_guard<std::mutex> lk(t->rnear_cache.mu);
+    t->rnear_cache.chunks_per_wave = 1.25 * (double)used / (double)ent->nw;
+    t->rnear_cache.last = std::move(ent);
+  }
 }
 
 template <typename D, int DIM>
@@ -977,10 +1182,45 @@ void rnear_fill_run(const pptk_tree* t, const pptk_queries* qs, double dmax, con
   oi.init(out_idx, (size_t)total * 4, out_mem);
   if (total > 0 && !out_idx) fail(PPTK_E_VALUE, "null output");
   const TreeView<D, DIM> tv = tree_view<D, DIM>(t);
-  // wave order of the fill; held until the stream has synchronised below
-  DevBuf wkeys, wvals;
+  const int pk = kRnearFillWidth;
+  // the event lists of the count of these queries, if it was the last count
+  // on this tree (held until the stream has synchronised below)
+  std::shared_ptr<RnearEvents> ent;
+  if (tv.nroots == 1 && pq.m > 0 && total > 0) {
+    std::lock_guard<std::mutex> lk(t->rnear_cache.mu);
+    const std::shared_ptr<RnearEvents>& e = t->rnear_cache.last;
+    if (e && e->src == pq.src && e->m == pq.m && e->mode == pq.mode && e->r == (double)r &&
+        e->pk % pk == 0)
+      ent = e;
+  }
+  // wave order of the fill and the replay check; held until the stream has
+  // synchronised below
+  DevBuf wkeys, wvals, same;
+  RnearEventView ev;
+  if (ent) {
+    // the count's processing order: a lane of the fill serves the query of
+    // the same lane of its count wave
+    pq.order = ent->order ? ent->order->as<const int32_t>() : nullptr;
+    same.reset(4);
+    fill_async<int32_t>(same.as<int32_t>(), 1, 1, s);
+    if (ent->qbytes) {
+      const int64_t words = (int64_t)(ent->qbytes / 4);
+      const unsigned g = (unsigned)std::min<int64_t>(4096, std::max<int64_t>(1, ceil_div(words, kBlock)));
+      words_equal_kernel<<<g, kBlock, 0, s>>>(
+          ent->queries.as<const uint32_t>(),
+          (const uint32_t*)(pq.mode == kQueryPoints ? pq.dpts : (const void*)pq.dlist), words,
+          same.as<int32_t>());
+      PK_LAUNCH_CHECK();
+    }
+    ev.node = ent->node.as<int32_t>();
+    ev.mask = ent->mask.as<uint64_t>();
+    ev.link = ent->link.as<int32_t>();
+    ev.hdr = ent->hdr.as<int32_t>();
+    ev.nchunks = (int32_t)ent->nchunks;
+    ev.pk = ent->pk;
+    ev.same = same.as<const int32_t>();
+  }
   if (pq.m > 0 && total > 0) {
-    const int pk = kRnearFillWidth;
     const int64_t nw = ceil_div(pq.m, pk);
     const unsigned grid = (unsigned)std::max<int64_t>(1, ceil_div(nw, kBlock / 64));
     // waves with the most output start first, so that the last waves to
@@ -1001,11 +1241,11 @@ void rnear_fill_run(const pptk_tree* t, const pptk_queries* qs, double dmax, con
     if (tv.nroots == 1 && pq.mode == kQueryPoints)
       rnear_packet_kernel<D, DIM, true, false><<<grid, kBlock, 0, s>>>(
           tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.order, pq.m, nrows, r2, pk, nw, worder,
-          nullptr, (const int64_t*)doff.ptr, total, (int32_t*)oi.dptr);
+          nullptr, (const int64_t*)doff.ptr, total, (int32_t*)oi.dptr, ev, 1);
     else if (tv.nroots == 1)
       rnear_packet_kernel<D, DIM, true, true><<<grid, kBlock, 0, s>>>(
           tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.order, pq.m, nrows, r2, pk, nw, worder,
-          nullptr, (const int64_t*)doff.ptr, total, (int32_t*)oi.dptr);
+          nullptr, (const int64_t*)doff.ptr, total, (int32_t*)oi.dptr, ev, 1);
     else
       rnear_kernel<D, DIM, true><<<grid_of(pq.m), kBlock, 0, s>>>(
           tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.order, pq.m, nrows, r2, nullptr,
@@ -1014,6 +1254,11 @@ void rnear_fill_run(const pptk_tree* t, const pptk_queries* qs, double dmax, con
   }
   oi.finish(s);
   PK_HIP(hipStreamSynchronize(s));
+  if (ent) {
+    // one replay per count: the lists are dropped once used
+    std::lock_guard<std::mutex> lk(t->rnear_cache.mu);
+    if (t->rnear_cache.last == ent) t->rnear_cache.last.reset();
+  }
 }
 
 void check_dmax(double dmax) {

@@ -17,6 +17,35 @@ struct OrderCache {
   int mode = -1;
   std::shared_ptr<DevBuf> order;
 };
+
+// The event lists of the last r-near count on a single-cloud tree (the nodes
+// at which each wave of queries took a whole range or scanned a leaf), kept
+// for the fill of the same queries at the same radius so that the fill does
+// not walk the tree again. The entry keeps a copy of the query data and the
+// processing order the count used; the fill replays it only if its own
+// queries equal the copy (compared on the device), and walks the tree
+// otherwise. An entry is published after the stream that made it has
+// synchronised and is never written afterwards; the fill that replays it
+// removes it from the cache.
+struct RnearEvents {
+  const void* src = nullptr;   // the caller's query array
+  int64_t m = -1;
+  int mode = -1;
+  double r = -1.0;             // radius in the tree's distance type
+  int pk = 0;                  // queries per count wave
+  int64_t nw = 0;              // count waves
+  int64_t nchunks = 0;         // chunks of 64 events in the pool
+  int64_t chunks_used = 0;     // chunks the count took (> nchunks: some waves not kept)
+  std::shared_ptr<DevBuf> order;  // processing order of the count (null: input order)
+  DevBuf queries;              // the query data the count read (points or indices)
+  size_t qbytes = 0;
+  DevBuf node, mask, link, hdr, head;
+};
+struct RnearCache {
+  std::mutex mu;
+  std::shared_ptr<RnearEvents> last;
+  double chunks_per_wave = 0;  // pool size the last count on this tree needed
+};
 }  // namespace pk
 
 // Host-side tree handle (opaque `pptk_tree` of the C ABI).
@@ -44,6 +73,7 @@ struct pptk_tree {
   pk::DevBuf node;  // NodeRec[nn]
   pk::DevBuf leafof;  // int32[n_valid]: leaf node holding each tree position
   mutable pk::OrderCache order_cache;
+  mutable pk::RnearCache rnear_cache;
 };
 
 namespace pk {

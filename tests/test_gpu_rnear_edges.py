@@ -21,7 +21,8 @@ pytestmark = pytest.mark.gpu
 
 def _count(tree, q, dmax):
     from pptk_amd import capi
-    off = np.empty(q.m + 1, np.int64)
+    m = tree.info.num_valid if q.kind == capi.Q_ALL else q.m
+    off = np.empty(m + 1, np.int64)
     tot = ctypes.c_int64()
     capi.check(capi.lib().pptk_rnear_count(ctypes.c_void_p(tree.handle), ctypes.byref(q),
                                            float(dmax), capi._p(off), ctypes.byref(tot),
@@ -123,3 +124,81 @@ def test_interleaved_counts_and_another_dmax(gpu):
     woff, widx = ot.rnear(None, d1)
     _same_sets(off, np.concatenate(lists), woff, widx)
     tree.free()
+
+
+def _lidar_case(n, m, seed):
+    from pptk_amd import scenes
+    rng = np.random.default_rng(seed)
+    pts = scenes.lidar_scene(n, seed=seed)
+    qp = np.ascontiguousarray(pts[rng.choice(len(pts), m, replace=False)])
+    return pts, qp
+
+
+@pytest.mark.parametrize("kind", ["points", "indices", "all"])
+def test_replayed_fill_equals_walked_fill(gpu, kind):
+    """The first fill after a count replays the count's event lists; they are
+    dropped once used, so a second fill with the same offsets walks the tree.
+    Both write every list in the same order (tree order), element for
+    element, and hold the oracle's sets."""
+    from pptk_amd import capi
+    pts, qp = _lidar_case(300_000, 30_000, 44)
+    tree = capi.Tree.build(pts)
+    if kind == "points":
+        src = qp
+    elif kind == "indices":
+        src = np.random.default_rng(45).choice(len(pts), 30_000, replace=False).astype(np.int32)
+    else:
+        src = None
+    q, keep = capi.make_queries(src)
+    dmax = float(np.float32(0.1))
+    off = _count(tree, q, dmax)
+    replayed = _fill(tree, q, dmax, off)
+    walked = _fill(tree, q, dmax, off)
+    assert off[-1] > 1_000_000
+    assert np.array_equal(replayed, walked)
+    woff, widx = O.OracleTree(pts).rnear(src, dmax)
+    _same_sets(off, replayed, woff, widx)
+    tree.free()
+
+
+@pytest.mark.parametrize("chunks", ["0", "1", "37"])
+def test_fill_when_the_event_pool_runs_out(gpu, monkeypatch, chunks):
+    """PPTK_RNEAR_EVENT_CHUNKS caps the event pool of a count: with no chunks
+    no list is kept, with too few the waves that could not take a chunk keep
+    none and their fill waves walk the tree; the rest replay. Results equal
+    those of an uncapped count and fill, element for element."""
+    from pptk_amd import capi
+    pts, qp = _lidar_case(200_000, 20_000, 46)
+    tree = capi.Tree.build(pts)
+    q, keep = capi.make_queries(qp)
+    dmax = float(np.float32(0.1))
+    off0 = _count(tree, q, dmax)
+    want = _fill(tree, q, dmax, off0)
+    monkeypatch.setenv("PPTK_RNEAR_EVENT_CHUNKS", chunks)
+    off = _count(tree, q, dmax)
+    got = _fill(tree, q, dmax, off)
+    monkeypatch.delenv("PPTK_RNEAR_EVENT_CHUNKS")
+    assert np.array_equal(off, off0)
+    assert np.array_equal(got, want)
+    tree.free()
+
+
+def test_fill_of_other_queries_at_the_same_address(gpu):
+    """A fill whose queries are not the count's (same array, new contents,
+    different per-row sizes) must not replay the count's lists: the fill's
+    offsets come from a count of the new contents made on another tree, so
+    the rows only line up if the fill answers the new contents."""
+    from pptk_amd import capi
+    pts, qp = _lidar_case(200_000, 20_000, 47)
+    tree = capi.Tree.build(pts)
+    other = capi.Tree.build(pts)
+    q, keep = capi.make_queries(qp)
+    dmax = float(np.float32(0.1))
+    _count(tree, q, dmax)
+    keep[:] = keep[::-1].copy()  # rows reversed in place
+    off = _count(other, q, dmax)
+    idx = _fill(tree, q, dmax, off)
+    woff, widx = O.OracleTree(pts).rnear(keep, dmax)
+    _same_sets(off, idx, woff, widx)
+    tree.free()
+    other.free()

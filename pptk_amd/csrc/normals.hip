@@ -311,43 +311,23 @@ __global__ __launch_bounds__(kBlock) void normals_rnear_kernel(TreeView<D, 3> tv
   bool ok;
   resolve_row<D>(tv, rows, t, q, row, pos, ok);
   if (row < 0) return;
-  // Shifted sums around q over the ball, in the precision P; returns n.
-  auto sums = [&](auto pc, double (&s1)[3], double (&s2)[6]) -> int64_t {
-    constexpr int P = decltype(pc)::value;
+  // Shifted sums around q over the ball, in FP64; returns n.
+  auto sums = [&](double (&s1)[3], double (&s2)[6]) -> int64_t {
     int64_t n = 0;
 #pragma unroll
     for (int i = 0; i < 3; ++i) s1[i] = 0.0;
 #pragma unroll
     for (int i = 0; i < 6; ++i) s2[i] = 0.0;
-    DF f1[3], f2[6];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) f1[i].zero();
-#pragma unroll
-    for (int i = 0; i < 6; ++i) f2[i].zero();
     if (!ok) return 0;
     auto visit = [&](const PointRec<D, 3>& r) {
       ++n;
-      if constexpr (P == PPTK_PREC_MIXED) {
-        float e[3];
+      double e[3];
 #pragma unroll
-        for (int a = 0; a < 3; ++a) { e[a] = (float)(r.c[a] - q[a]); f1[a].add(e[a]); }
-        f2[0].add_prod(e[0], e[0]); f2[1].add_prod(e[0], e[1]); f2[2].add_prod(e[0], e[2]);
-        f2[3].add_prod(e[1], e[1]); f2[4].add_prod(e[1], e[2]); f2[5].add_prod(e[2], e[2]);
-      } else {
-        double e[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) { e[a] = (double)r.c[a] - (double)q[a]; s1[a] = s1[a] + e[a]; }
-        s2[0] = s2[0] + e[0] * e[0]; s2[1] = s2[1] + e[0] * e[1]; s2[2] = s2[2] + e[0] * e[2];
-        s2[3] = s2[3] + e[1] * e[1]; s2[4] = s2[4] + e[1] * e[2]; s2[5] = s2[5] + e[2] * e[2];
-      }
+      for (int a = 0; a < 3; ++a) { e[a] = (double)r.c[a] - (double)q[a]; s1[a] = s1[a] + e[a]; }
+      s2[0] = s2[0] + e[0] * e[0]; s2[1] = s2[1] + e[0] * e[1]; s2[2] = s2[2] + e[0] * e[2];
+      s2[3] = s2[3] + e[1] * e[1]; s2[4] = s2[4] + e[1] * e[2]; s2[5] = s2[5] + e[2] * e[2];
     };
     rnear_visit<D, 3>(tv, root_of<D, 3>(tv, pos), q, r2, visit);
-    if constexpr (P == PPTK_PREC_MIXED) {
-#pragma unroll
-      for (int i = 0; i < 3; ++i) s1[i] = f1[i].value();
-#pragma unroll
-      for (int i = 0; i < 6; ++i) s2[i] = f2[i].value();
-    }
     return n;
   };
   auto covariance = [](int64_t n, const double (&s1)[3], const double (&s2)[6],
@@ -363,7 +343,7 @@ __global__ __launch_bounds__(kBlock) void normals_rnear_kernel(TreeView<D, 3> tv
     cd[1][0] = cd[0][1]; cd[2][0] = cd[0][2]; cd[2][1] = cd[1][2];
   };
   double s1[3], s2[6], cd[3][3];
-  int64_t n = sums(std::integral_constant<int, PREC>{}, s1, s2);
+  const int64_t n = sums(s1, s2);
   bool fb = false;
   if (n == 0) {
     write_nan<D, NE>(row, evecs, evals, sizes);
@@ -382,10 +362,7 @@ __global__ __launch_bounds__(kBlock) void normals_rnear_kernel(TreeView<D, 3> tv
         write_eig<D, float, NE>(row, w, v, (int)n, evecs, evals, sizes);
         solved = true;
       } else {
-        // below the gap threshold: the FP64 mode's sums over the same ball
-        n = sums(std::integral_constant<int, PPTK_PREC_FP64>{}, s1, s2);
-        covariance(n, s1, s2, cd);
-        fb = true;
+        fb = true;  // below the gap threshold: the FP64 solve of the same covariance
       }
     }
     if (!solved) {

@@ -134,16 +134,18 @@ def _lidar_case(n, m, seed):
     return pts, qp
 
 
-@pytest.mark.parametrize("kind", ["points", "indices", "all"])
+@pytest.mark.parametrize("kind", ["points", "indices", "all", "points64"])
 def test_replayed_fill_equals_walked_fill(gpu, kind):
-    """The first fill after a count replays the count's event lists; they are
-    dropped once used, so a second fill with the same offsets walks the tree.
-    Both write every list in the same order (tree order), element for
-    element, and hold the oracle's sets."""
+    """The first fill after a count replays the count's event lists (float32
+    and float64 trees); they are dropped once used, so a second fill with the
+    same offsets walks the tree. Both write every list in the same order
+    (tree order), element for element, and hold the oracle's sets."""
     from pptk_amd import capi
     pts, qp = _lidar_case(300_000, 30_000, 44)
+    if kind == "points64":
+        pts, qp = pts.astype(np.float64), qp.astype(np.float64)
     tree = capi.Tree.build(pts)
-    if kind == "points":
+    if kind in ("points", "points64"):
         src = qp
     elif kind == "indices":
         src = np.random.default_rng(45).choice(len(pts), 30_000, replace=False).astype(np.int32)

@@ -210,9 +210,9 @@ pptk_status pptk_rnear_fill(const pptk_tree* tree, const pptk_queries* q, double
 /* Normal estimation precision modes. */
 typedef enum {
   PPTK_PREC_AUTO = 0,  /* FP64 for float64 points, MIXED for float32 points */
-  PPTK_PREC_MIXED = 1, /* FP32 covariance centred on the query point with
-                          compensated (double-float) sums; FP32 eigensolve;
-                          FP64 eigensolve where (l1-l0)/l2 < 1e-3 */
+  PPTK_PREC_MIXED = 1, /* FP64 covariance centred on the query point; FP32
+                          eigensolve; FP64 eigensolve of the same covariance
+                          where (l1-l0)/l2 < 1e-3 */
   PPTK_PREC_FP64 = 2   /* FP64 covariance and eigensolve */
 } pptk_precision;
 

@@ -8,16 +8,14 @@
 // (T)r) or RNearNeighbors(q, (T)r), X -= colwise mean, C = X^T X / n,
 // Eigen::SelfAdjointEigenSolver, eigenvalues ascending.
 //
-// Precision modes:
-//   MIXED : neighbour offsets e = p - q relative to the query point (the
-//           centroid shift), mean and covariance accumulated in FP32 as
-//           double-float pairs (TwoSum for additions, FMA-exact TwoProduct
-//           for products; Ogita, Rump & Oishi, "Accurate sum and dot
-//           product", SIAM J. Sci. Comput. 26(6), 2005), FP32 Jacobi; rows
-//           whose eigengap (l1 - l0) / l2 < 1e-3 (or, with all eigenvectors,
+// Precision modes (both form the covariance in FP64: neighbour offsets
+// e = p - q relative to the query point, their mean, the centred products;
+// FP64 adds and multiplies issue at the FP32 rate on this hardware):
+//   MIXED : FP32 Jacobi of the covariance rounded to float; rows whose
+//           eigengap (l1 - l0) / l2 < 1e-3 (or, with all eigenvectors,
 //           (l2 - l1) / l2 < 1e-3) are re-solved in FP64 from the same
-//           double-float covariance.
-//   FP64  : offsets, covariance and Jacobi in double.
+//           covariance, so they equal the FP64 mode's rows.
+//   FP64  : Jacobi in double.
 #include <algorithm>
 #include <type_traits>
 

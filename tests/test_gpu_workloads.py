@@ -4,8 +4,8 @@ Precision stress: the 10M-point lidar scene in float64, translated by
 (512000, 5400000, 300) m, plus a 2M-point near-isotropic cluster (sigma
 0.05 m) made of isotropic 16-point groups, whose neighbourhoods have
 eigengaps (l1 - l0) / l2 from ~1e-6 to ~1e-2 (pptk_amd.scenes). Normals k=16
-in the mixed mode (FP32 centred double-float covariance; rows below the 1e-3
-eigengap threshold recomputed and solved in FP64) are compared with FP64
+in the mixed mode (FP64 centred covariance, FP32 eigensolve; rows below the
+1e-3 eigengap threshold solved in FP64) are compared with FP64
 normals of the same neighbourhoods (oracle kNN + LAPACK) on seeded samples of
 scene and cluster rows, every sampled row within 0.5 degrees, the tiny-gap
 rows included. The angle-error distribution and the FP64-fallback fraction

@@ -381,8 +381,8 @@ struct RnearEventView {
 //     and tested against each straddling lane's query in turn, the hits
 //     ranked with a ballot; at an inner node the wave descends.
 // Each lane's output is in tree order, exactly as rnear_kernel's.
-// The count records its events when ev.node is set (pk <= 32); the fill
-// replays them when ev.hdr is set, *ev.same is nonzero and its count wave
+// The count records its events when ev_node is set (pk <= 32); the fill
+// replays them when ev_hdr is set, *ev.same is nonzero and its count wave
 // kept its list, and walks the tree otherwise.
 // EXCL: self queries (tree points): the query's own position is skipped.
 // Lane values needed by the whole wave (query, own position, write pointer
@@ -400,7 +400,16 @@ __global__ __launch_bounds__(kBlock) void rnear_packet_kernel(TreeView<D, DIM> t
                                                               const int64_t* __restrict__ offsets,
                                                               int64_t total,
                                                               int32_t* __restrict__ out,
-                                                              RnearEventView ev, int xrun) {
+                                                              RnearEventView ev,
+                                                              int32_t* __restrict__ ev_node,
+                                                              uint64_t* __restrict__ ev_mask,
+                                                              int32_t* __restrict__ ev_link,
+                                                              int32_t* __restrict__ ev_hdr,
+                                                              uint32_t* __restrict__ ev_head,
+                                                              int xrun) {
+  // The event arrays come in as restrict parameters (the same pointers as
+  // ev's): the count's stores to them then cannot alias the tree, whose
+  // node, box and point reads stay scalar loads.
   const int lane = threadIdx.x & 63;
   int64_t w = xcd_block(blockIdx.x, gridDim.x, xrun) * (kBlock / 64) + (threadIdx.x >> 6);
   if (wave_order && w < nw) w = wave_order[w];
@@ -516,13 +525,13 @@ __global__ __launch_bounds__(kBlock) void rnear_packet_kernel(TreeView<D, DIM> t
 
   bool walk = true;
   if constexpr (FILL) {
-    if (ev.hdr && actm) {
+    if (ev_hdr && actm) {
       // this wave's lanes are lanes h0 .. h0 + pk - 1 of count wave cw
       const int64_t t0 = w * pk;
       const int64_t cw = t0 / ev.pk;
       const int h0 = (int)(t0 - cw * ev.pk);
-      const int32_t first = ev.hdr[2 * cw];
-      const int32_t nev = ev.hdr[2 * cw + 1];
+      const int32_t first = ev_hdr[2 * cw];
+      const int32_t nev = ev_hdr[2 * cw + 1];
       if (*ev.same != 0 && nev >= 0) {
         walk = false;
         const uint32_t lm = pk >= 32 ? 0xffffffffu : ((1u << pk) - 1u);
@@ -530,12 +539,12 @@ __global__ __launch_bounds__(kBlock) void rnear_packet_kernel(TreeView<D, DIM> t
         for (int32_t e0 = 0; e0 < nev; e0 += kEventChunk) {
           const bool has = lane < nev - e0;
           const int64_t slot = (int64_t)chunk * kEventChunk + lane;
-          const int32_t en = has ? ev.node[slot] : 0;
-          const uint64_t em = has ? ev.mask[slot] : 0ull;
+          const int32_t en = has ? ev_node[slot] : 0;
+          const uint64_t em = has ? ev_mask[slot] : 0ull;
           const uint32_t mi = ((uint32_t)em >> h0) & lm;
           const uint32_t ms = ((uint32_t)(em >> 32) >> h0) & lm;
           uint64_t todo = __ballot((mi | ms) != 0u);
-          const int32_t next = e0 + kEventChunk < nev ? ev.link[chunk] : -1;
+          const int32_t next = e0 + kEventChunk < nev ? ev_link[chunk] : -1;
           while (todo) {
             const int j = __builtin_ctzll(todo);
             todo &= todo - 1ull;
@@ -558,13 +567,13 @@ __global__ __launch_bounds__(kBlock) void rnear_packet_kernel(TreeView<D, DIM> t
     // the count's event recording (uniform state; lane j of en/em holds
     // event j of the batch not yet written)
     bool rec = false;
-    if constexpr (!FILL) rec = ev.node != nullptr && w < nw;
+    if constexpr (!FILL) rec = ev_node != nullptr && w < nw;
     int32_t en = 0;
     uint64_t em = 0;
     int32_t e_pos = 0, e_first = -1, e_chunk = -1, e_count = 0;
     auto flush = [&]() {
       int32_t c = 0;
-      if (lane == 0) c = (int32_t)atomicAdd(ev.head, 1u);
+      if (lane == 0) c = (int32_t)atomicAdd(ev_head, 1u);
       c = __builtin_amdgcn_readlane(c, 0);
       if (c < 0 || c >= ev.nchunks) {
         rec = false;  // pool exhausted: the fill walks for this wave
@@ -572,11 +581,11 @@ __global__ __launch_bounds__(kBlock) void rnear_packet_kernel(TreeView<D, DIM> t
       }
       const int64_t slot = (int64_t)c * kEventChunk + lane;
       if (lane < e_pos) {
-        ev.node[slot] = en;
-        ev.mask[slot] = em;
+        ev_node[slot] = en;
+        ev_mask[slot] = em;
       }
       if (e_chunk >= 0) {
-        if (lane == 0) ev.link[e_chunk] = c;
+        if (lane == 0) ev_link[e_chunk] = c;
       } else {
         e_first = c;
       }
@@ -624,11 +633,11 @@ __global__ __launch_bounds__(kBlock) void rnear_packet_kernel(TreeView<D, DIM> t
       node = descend ? nr.child : next_in_order<D, DIM>(tv, node);
     }
     if constexpr (!FILL) {
-      if (ev.node != nullptr && w < nw) {
+      if (ev_node != nullptr && w < nw) {
         if (rec && e_pos > 0) flush();
         if (lane == 0) {
-          ev.hdr[2 * w] = e_first;
-          ev.hdr[2 * w + 1] = rec ? e_count : -1;
+          ev_hdr[2 * w] = e_first;
+          ev_hdr[2 * w + 1] = rec ? e_count : -1;
         }
       }
     }
@@ -1128,11 +1137,11 @@ void rnear_count_run(const pptk_tree* t, const pptk_queries* qs, double dmax, in
     if (tv.nroots == 1 && pq.mode == kQueryPoints)
       rnear_packet_kernel<D, DIM, false, false><<<grid, kBlock, 0, s>>>(
           tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.order, pq.m, nrows, r2, pk, nw, nullptr,
-          counts.as<int32_t>(), nullptr, 0, nullptr, ev, kXcdRun);
+          counts.as<int32_t>(), nullptr, 0, nullptr, ev, ev.node, ev.mask, ev.link, ev.hdr, ev.head, kXcdRun);
     else if (tv.nroots == 1)
       rnear_packet_kernel<D, DIM, false, true><<<grid, kBlock, 0, s>>>(
           tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.order, pq.m, nrows, r2, pk, nw, nullptr,
-          counts.as<int32_t>(), nullptr, 0, nullptr, ev, kXcdRun);
+          counts.as<int32_t>(), nullptr, 0, nullptr, ev, ev.node, ev.mask, ev.link, ev.hdr, ev.head, kXcdRun);
     else
       rnear_kernel<D, DIM, false><<<grid_of(pq.m), kBlock, 0, s>>>(
           tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.order, pq.m, nrows, r2,
@@ -1241,11 +1250,11 @@ void rnear_fill_run(const pptk_tree* t, const pptk_queries* qs, double dmax, con
     if (tv.nroots == 1 && pq.mode == kQueryPoints)
       rnear_packet_kernel<D, DIM, true, false><<<grid, kBlock, 0, s>>>(
           tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.order, pq.m, nrows, r2, pk, nw, worder,
-          nullptr, (const int64_t*)doff.ptr, total, (int32_t*)oi.dptr, ev, 1);
+          nullptr, (const int64_t*)doff.ptr, total, (int32_t*)oi.dptr, ev, ev.node, ev.mask, ev.link, ev.hdr, ev.head, 1);
     else if (tv.nroots == 1)
       rnear_packet_kernel<D, DIM, true, true><<<grid, kBlock, 0, s>>>(
           tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.order, pq.m, nrows, r2, pk, nw, worder,
-          nullptr, (const int64_t*)doff.ptr, total, (int32_t*)oi.dptr, ev, 1);
+          nullptr, (const int64_t*)doff.ptr, total, (int32_t*)oi.dptr, ev, ev.node, ev.mask, ev.link, ev.hdr, ev.head, 1);
     else
       rnear_kernel<D, DIM, true><<<grid_of(pq.m), kBlock, 0, s>>>(
           tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.order, pq.m, nrows, r2, nullptr,

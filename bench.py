@@ -328,18 +328,21 @@ def bench_lidar(args):
         rec = {}
         for _ in range(warmup):
             p.run(None)
+        # the phase events are created before and read after the timed steps
+        # (every library call returns with its stream synchronised)
+        evs = [[torch.cuda.Event(enable_timing=True) for _ in range(len(phases) + 1)]
+               for _ in range(steps)]
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for i in range(steps):
-            ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(phases) + 1)]
-            p.run(ev)
-            p.stream.synchronize()
+            p.run(evs[i])
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        for i, ev in enumerate(evs):
             for j, name in enumerate(phases):
                 rec.setdefault(name, []).append(ev[j].elapsed_time(ev[j + 1]))
             if say:
-                log("bench: step %d done" % i)
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
+                log("bench: step %d: %.3f ms of phases" % (i, ev[0].elapsed_time(ev[-1])))
         return wall * 1e3 / steps, {q: float(np.mean(v)) for q, v in rec.items()}
 
     log("bench: warmup %d, timing %d steps" % (args.warmup, args.steps))
@@ -468,21 +471,21 @@ def bench_batched(args, plain_10m_pps=None):
     run_args = (pts_d.data_ptr(), idx_d.data_ptr(), None, nrm_d.data_ptr(), None)
     for _ in range(args.warmup):
         plan.run(*run_args, stream=sh)
+    # events created before and read after the timed replays
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(args.steps)]
     torch.cuda.synchronize()
-    times = []
     t0 = time.perf_counter()
     for i in range(args.steps):
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
+        evs[i][0].record(stream)
         plan.run(*run_args, stream=sh)
-        e1.record(stream)
-        stream.synchronize()
-        times.append(e0.elapsed_time(e1))
-        log("bench: batched step %d %.2f ms" % (i, times[-1]))
+        evs[i][1].record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_step = wall * 1e3 / args.steps
+    times = [e0.elapsed_time(e1) for e0, e1 in evs]
+    for i, t in enumerate(times):
+        log("bench: batched step %d %.2f ms" % (i, t))
     binfo = plan.info()
 
     # phase split from one eager timed run (same kernels, no graph)

@@ -204,3 +204,51 @@ def test_fill_of_other_queries_at_the_same_address(gpu):
     _same_sets(off, idx, woff, widx)
     tree.free()
     other.free()
+
+
+def test_concurrent_counts_and_fills_on_one_tree(gpu):
+    """Two host threads, each on its own stream with its own query set, run
+    count then fill on the same tree over and over: each thread's count
+    replaces the tree's event lists, so a fill replays its own count's lists
+    or finds the other thread's and walks the tree. Every result must hold the
+    oracle's sets."""
+    import threading
+
+    import torch
+
+    from pptk_amd import capi
+    pts, _ = _lidar_case(200_000, 1, 48)
+    tree = capi.Tree.build(pts)
+    ot = O.OracleTree(pts)
+    rng = np.random.default_rng(49)
+    sets = [np.ascontiguousarray(pts[rng.choice(len(pts), 12_000, replace=False)]) for _ in range(2)]
+    dmax = float(np.float32(0.1))
+    want = [ot.rnear(s, dmax) for s in sets]
+    errors = []
+
+    def worker(i):
+        try:
+            stream = torch.cuda.Stream()
+            sh = ctypes.c_void_p(stream.cuda_stream)
+            q, keep = capi.make_queries(sets[i])
+            for _ in range(6):
+                off = np.empty(q.m + 1, np.int64)
+                tot = ctypes.c_int64()
+                capi.check(capi.lib().pptk_rnear_count(ctypes.c_void_p(tree.handle), ctypes.byref(q),
+                                                       dmax, capi._p(off), ctypes.byref(tot),
+                                                       capi.MEM_HOST, sh))
+                idx = np.empty(max(int(off[-1]), 1), np.int32)
+                capi.check(capi.lib().pptk_rnear_fill(ctypes.c_void_p(tree.handle), ctypes.byref(q),
+                                                      dmax, capi._p(off), capi._p(idx),
+                                                      capi.MEM_HOST, sh))
+                _same_sets(off, idx[: int(off[-1])], *want[i])
+        except Exception as e:  # reported by the main thread
+            errors.append(e)
+
+    threads = [threading.Thread(target=worker, args=(i,)) for i in range(2)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    tree.free()
+    assert not errors, errors[0]

@@ -199,7 +199,13 @@ pptk_status pptk_knn_noalloc(const pptk_tree* tree, const pptk_queries* q, int k
  * behind kdtree._query(k=None). Phase 1 writes CSR offsets (m+1 int64,
  * offsets[m] = total) and the total into *out_total (host). Phase 2 writes
  * the neighbour lists (original indices) given those offsets. Within a list,
- * neighbours appear in tree order; compare lists as sets. */
+ * neighbours appear in tree order; compare lists as sets. A list is written
+ * only inside [offsets[i], offsets[i+1]) clamped to [0, total). With host
+ * offsets a negative total is an error; with offsets and out_idx in device
+ * memory the kernels read the total themselves (no wait for a copy to the
+ * host) and a negative total writes nothing. The fill right after the count
+ * of the same queries at the same dmax replays the count's traversal instead
+ * of walking the tree again (same output). */
 pptk_status pptk_rnear_count(const pptk_tree* tree, const pptk_queries* q, double dmax,
                              int64_t* out_offsets, int64_t* out_total, pptk_mem out_mem,
                              void* stream);

@@ -188,8 +188,11 @@ __global__ __launch_bounds__(kBlock) void rnear_kernel(TreeView<D, DIM> tv, int 
                                                        int64_t m, int64_t nrows, D r2,
                                                        int32_t* __restrict__ counts,
                                                        const int64_t* __restrict__ offsets,
-                                                       int64_t total,
+                                                       int64_t total_arg,
+                                                       const int64_t* __restrict__ total_dev,
                                                        int32_t* __restrict__ out) {
+  // the fill's output length: total_arg, or read here when given on the device
+  const int64_t total = total_dev ? (*total_dev > 0 ? *total_dev : 0) : total_arg;
   const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const int lane = threadIdx.x & 63;
   const uint64_t lt_mask = (1ull << lane) - 1ull;
@@ -398,7 +401,8 @@ __global__ __launch_bounds__(kBlock) void rnear_packet_kernel(TreeView<D, DIM> t
                                                               const uint32_t* __restrict__ wave_order,
                                                               int32_t* __restrict__ counts,
                                                               const int64_t* __restrict__ offsets,
-                                                              int64_t total,
+                                                              int64_t total_arg,
+                                                              const int64_t* __restrict__ total_dev,
                                                               int32_t* __restrict__ out,
                                                               RnearEventView ev,
                                                               int32_t* __restrict__ ev_node,
@@ -410,6 +414,12 @@ __global__ __launch_bounds__(kBlock) void rnear_packet_kernel(TreeView<D, DIM> t
   // The event arrays come in as restrict parameters (the same pointers as
   // ev's): the count's stores to them then cannot alias the tree, whose
   // node, box and point reads stay scalar loads.
+  // The fill's output length: total_arg, or read here when given on the
+  // device (a negative total writes nothing).
+  int64_t total = total_arg;
+  if constexpr (FILL) {
+    if (total_dev) total = *total_dev > 0 ? *total_dev : 0;
+  }
   const int lane = threadIdx.x & 63;
   int64_t w = xcd_block(blockIdx.x, gridDim.x, xrun) * (kBlock / 64) + (threadIdx.x >> 6);
   if (wave_order && w < nw) w = wave_order[w];
@@ -1137,15 +1147,15 @@ void rnear_count_run(const pptk_tree* t, const pptk_queries* qs, double dmax, in
     if (tv.nroots == 1 && pq.mode == kQueryPoints)
       rnear_packet_kernel<D, DIM, false, false><<<grid, kBlock, 0, s>>>(
           tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.order, pq.m, nrows, r2, pk, nw, nullptr,
-          counts.as<int32_t>(), nullptr, 0, nullptr, ev, ev.node, ev.mask, ev.link, ev.hdr, ev.head, kXcdRun);
+          counts.as<int32_t>(), nullptr, 0, nullptr, nullptr, ev, ev.node, ev.mask, ev.link, ev.hdr, ev.head, kXcdRun);
     else if (tv.nroots == 1)
       rnear_packet_kernel<D, DIM, false, true><<<grid, kBlock, 0, s>>>(
           tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.order, pq.m, nrows, r2, pk, nw, nullptr,
-          counts.as<int32_t>(), nullptr, 0, nullptr, ev, ev.node, ev.mask, ev.link, ev.hdr, ev.head, kXcdRun);
+          counts.as<int32_t>(), nullptr, 0, nullptr, nullptr, ev, ev.node, ev.mask, ev.link, ev.hdr, ev.head, kXcdRun);
     else
       rnear_kernel<D, DIM, false><<<grid_of(pq.m), kBlock, 0, s>>>(
           tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.order, pq.m, nrows, r2,
-          counts.as<int32_t>(), nullptr, 0, nullptr);
+          counts.as<int32_t>(), nullptr, 0, nullptr, nullptr);
     PK_LAUNCH_CHECK();
     if (ent) PK_HIP(hipMemcpyAsync(&used, ent->head.get(), 4, hipMemcpyDeviceToHost, s));
   }
@@ -1180,11 +1190,18 @@ void rnear_fill_run(const pptk_tree* t, const pptk_queries* qs, double dmax, con
   const D r2 = r * r;
   const int64_t nrows = pq.nrows;
   if (!offsets) fail(PPTK_E_VALUE, "null offsets");
+  // With offsets and output in device memory the kernels read the total
+  // (offsets[nrows]) themselves: no copy to the host and no wait for it.
+  const bool dev_total = out_mem == PPTK_MEM_DEVICE && out_idx != nullptr;
   int64_t total = 0;
-  if (out_mem == PPTK_MEM_HOST) total = offsets[nrows];
-  else PK_HIP(hipMemcpyAsync(&total, offsets + nrows, 8, hipMemcpyDeviceToHost, s));
-  PK_HIP(hipStreamSynchronize(s));
-  if (total < 0) fail(PPTK_E_VALUE, "negative total in offsets");
+  if (!dev_total) {
+    if (out_mem == PPTK_MEM_HOST) total = offsets[nrows];
+    else PK_HIP(hipMemcpyAsync(&total, offsets + nrows, 8, hipMemcpyDeviceToHost, s));
+    PK_HIP(hipStreamSynchronize(s));
+    if (total < 0) fail(PPTK_E_VALUE, "negative total in offsets");
+  }
+  const int64_t* total_dev = dev_total ? offsets + nrows : nullptr;
+  const bool work = dev_total || total > 0;
   DevInput doff;
   stage_input(doff, offsets, (size_t)(nrows + 1) * 8, out_mem, s);
   DevOutput oi;
@@ -1195,7 +1212,7 @@ void rnear_fill_run(const pptk_tree* t, const pptk_queries* qs, double dmax, con
   // the event lists of the count of these queries, if it was the last count
   // on this tree (held until the stream has synchronised below)
   std::shared_ptr<RnearEvents> ent;
-  if (tv.nroots == 1 && pq.m > 0 && total > 0) {
+  if (tv.nroots == 1 && pq.m > 0 && work) {
     std::lock_guard<std::mutex> lk(t->rnear_cache.mu);
     const std::shared_ptr<RnearEvents>& e = t->rnear_cache.last;
     if (e && e->src == pq.src && e->m == pq.m && e->mode == pq.mode && e->r == (double)r &&
@@ -1229,7 +1246,7 @@ void rnear_fill_run(const pptk_tree* t, const pptk_queries* qs, double dmax, con
     ev.pk = ent->pk;
     ev.same = same.as<const int32_t>();
   }
-  if (pq.m > 0 && total > 0) {
+  if (pq.m > 0 && work) {
     const int64_t nw = ceil_div(pq.m, pk);
     const unsigned grid = (unsigned)std::max<int64_t>(1, ceil_div(nw, kBlock / 64));
     // waves with the most output start first, so that the last waves to
@@ -1250,15 +1267,17 @@ void rnear_fill_run(const pptk_tree* t, const pptk_queries* qs, double dmax, con
     if (tv.nroots == 1 && pq.mode == kQueryPoints)
       rnear_packet_kernel<D, DIM, true, false><<<grid, kBlock, 0, s>>>(
           tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.order, pq.m, nrows, r2, pk, nw, worder,
-          nullptr, (const int64_t*)doff.ptr, total, (int32_t*)oi.dptr, ev, ev.node, ev.mask, ev.link, ev.hdr, ev.head, 1);
+          nullptr, (const int64_t*)doff.ptr, total, total_dev, (int32_t*)oi.dptr, ev, ev.node,
+          ev.mask, ev.link, ev.hdr, ev.head, 1);
     else if (tv.nroots == 1)
       rnear_packet_kernel<D, DIM, true, true><<<grid, kBlock, 0, s>>>(
           tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.order, pq.m, nrows, r2, pk, nw, worder,
-          nullptr, (const int64_t*)doff.ptr, total, (int32_t*)oi.dptr, ev, ev.node, ev.mask, ev.link, ev.hdr, ev.head, 1);
+          nullptr, (const int64_t*)doff.ptr, total, total_dev, (int32_t*)oi.dptr, ev, ev.node,
+          ev.mask, ev.link, ev.hdr, ev.head, 1);
     else
       rnear_kernel<D, DIM, true><<<grid_of(pq.m), kBlock, 0, s>>>(
           tv, pq.mode, pq.dlist, (const D*)pq.dpts, pq.order, pq.m, nrows, r2, nullptr,
-          (const int64_t*)doff.ptr, total, (int32_t*)oi.dptr);
+          (const int64_t*)doff.ptr, total, total_dev, (int32_t*)oi.dptr);
     PK_LAUNCH_CHECK();
   }
   oi.finish(s);

@@ -213,21 +213,26 @@ __global__ __launch_bounds__(kBlock) void ingest_finalize_kernel(const uint32_t*
   }
 }
 
-// Stable compaction of a tile's valid rows and their Morton codes. Rows are
-// read lane-consecutively (row e of the tile = points [e*kBlock, (e+1)*kBlock)
-// of it); a row's valid points are ranked with a wave ballot, and the
-// per-(row, wave) counts are scanned in (row, wave) order, which is index
-// order, so the compaction keeps the input order.
+// Stable compaction of a tile's valid rows and their Morton codes to the
+// front of the key array, and of its invalid (NaN/inf) rows, keyed
+// `sentinel` (every sorted key bit set), behind all n_valid valid ones: a
+// stable sort then leaves every invalid row after every valid row, so the
+// host need not know n_valid before sorting. Rows are read lane-consecutively
+// (row e of the tile = points [e*kBlock, (e+1)*kBlock) of it); a row's points
+// are ranked with wave ballots, and the per-(row, wave) counts are scanned in
+// (row, wave) order, which is index order, so the compaction keeps the input
+// order.
 template <typename T, typename D, int DIM>
 __global__ __launch_bounds__(kBlock) void morton_kernel(const T* __restrict__ pts, int64_t n,
                                                         const uint32_t* __restrict__ blk_off,
                                                         const BuildInfo<D>* __restrict__ info_p,
                                                         uint64_t* __restrict__ keys,
                                                         uint32_t* __restrict__ vals,
-                                                        int kshift) {
+                                                        int kshift, uint64_t sentinel) {
   constexpr int kWaves = kBlock / 64;
   static_assert(kPer * kWaves == 64, "one wave scans the (row, wave) counts");
   __shared__ uint32_t s_off[kPer * kWaves];
+  __shared__ uint32_t s_inv[kPer * kWaves];
   const BuildInfo<D> info = *info_p;
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
@@ -243,45 +248,71 @@ __global__ __launch_bounds__(kBlock) void morton_kernel(const T* __restrict__ pt
     }
     flags |= (uint32_t)ok << e;
     const uint64_t b = __ballot(ok);
-    if (lane == 0) s_off[e * kWaves + w] = (uint32_t)__popcll(b);
+    const uint64_t inr = __ballot(i < n);
+    if (lane == 0) {
+      s_off[e * kWaves + w] = (uint32_t)__popcll(b);
+      s_inv[e * kWaves + w] = (uint32_t)__popcll(inr & ~b);
+    }
   }
   __syncthreads();
   if (w == 0) {
-    // exclusive scan of the kPer * kWaves (= 64) counts, one per lane
-    const uint32_t c = s_off[lane];
-    uint32_t incl = c;
+    // exclusive scans of the kPer * kWaves (= 64) counts, one per lane
+    const uint32_t c = s_off[lane], ci = s_inv[lane];
+    uint32_t incl = c, incli = ci;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
       const uint32_t y = __shfl_up(incl, o);
-      if (lane >= o) incl += y;
+      const uint32_t yi = __shfl_up(incli, o);
+      if (lane >= o) {
+        incl += y;
+        incli += yi;
+      }
     }
     s_off[lane] = incl - c;
+    s_inv[lane] = incli - ci;
   }
   __syncthreads();
   const uint32_t tile0 = blk_off[blockIdx.x];
+  // invalid rows of the earlier tiles (all full): their rows minus their valid ones
+  const uint32_t inv0 = info.n_valid + (uint32_t)((int64_t)blockIdx.x * kTile - tile0);
   for (int e = 0; e < kPer; ++e) {
-    const bool ok = (flags >> e) & 1u;
-    const uint64_t b = __ballot(ok);
-    if (!ok) continue;
     const int64_t i = first + (int64_t)e * kBlock;
-    const uint32_t pos = tile0 + s_off[e * kWaves + w] + (uint32_t)__popcll(b & lt_mask);
-    D c[DIM];
+    const bool ok = (flags >> e) & 1u;
+    const bool bad = !ok && i < n;
+    const uint64_t b = __ballot(ok);
+    const uint64_t bb = __ballot(bad);
+    if (ok) {
+      const uint32_t pos = tile0 + s_off[e * kWaves + w] + (uint32_t)__popcll(b & lt_mask);
+      D c[DIM];
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) c[d] = (D)pts[i * DIM + d];
-    keys[pos] = morton_code<D, DIM>(c, info) >> kshift;
-    vals[pos] = (uint32_t)i;
+      for (int d = 0; d < DIM; ++d) c[d] = (D)pts[i * DIM + d];
+      keys[pos] = morton_code<D, DIM>(c, info) >> kshift;
+      vals[pos] = (uint32_t)i;
+    } else if (bad) {
+      const uint32_t pos = inv0 + s_inv[e * kWaves + w] + (uint32_t)__popcll(bb & lt_mask);
+      keys[pos] = sentinel;
+      vals[pos] = (uint32_t)i;
+    }
   }
 }
 
+// Tree-order records and both row maps from the sorted order of all n rows:
+// positions [0, n_valid) are the valid rows in tree order, the rest the
+// invalid rows, whose forward entry is -1.
 template <typename T, typename D, int DIM>
 __global__ __launch_bounds__(kBlock) void gather_kernel(const T* __restrict__ pts,
                                                         const uint32_t* __restrict__ order,
-                                                        int64_t nv, PointRec<D, DIM>* __restrict__ rec,
+                                                        int64_t n, const uint32_t* __restrict__ nv_dev,
+                                                        PointRec<D, DIM>* __restrict__ rec,
                                                         int32_t* __restrict__ orig,
                                                         int32_t* __restrict__ fwd) {
   const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (p >= nv) return;
+  if (p >= n) return;
   const uint32_t i = order[p];
+  if (p >= (int64_t)*nv_dev) {
+    fwd[i] = -1;
+    return;
+  }
   PointRec<D, DIM> r;
 #pragma unroll
   for (int d = 0; d < DIM; ++d) r.c[d] = (D)pts[(int64_t)i * DIM + d];
@@ -315,26 +346,18 @@ void build_impl(pptk_tree* t, const T* dpts, int64_t n, hipStream_t s) {
                                                       blk_off.as<uint32_t>(),
                                                       info_d.as<BuildInfo<D>>());
   PK_LAUNCH_CHECK();
-  BuildInfo<D> info;
-  PK_HIP(hipMemcpyAsync(&info, info_d.get(), sizeof(info), hipMemcpyDeviceToHost, s));
-  PK_HIP(hipStreamSynchronize(s));
-  const int64_t nv = info.n_valid;
-  t->n_valid = nv;
-  for (int d = 0; d < 4; ++d) {
-    t->lo[d] = d < DIM ? (double)info.lo[d] : 0.0;
-    t->hi[d] = d < DIM ? (double)info.hi[d] : 0.0;
-    t->mlo[d] = (double)info.lo[d];
-    t->mscale[d] = (double)info.scale[d];
-  }
+  // The valid-row count and the frame stay on the device until the level
+  // table is read back: every launch before that is sized by n (invalid rows
+  // sort behind the valid ones) and reads the count where it needs it.
+  const uint32_t* nv_dev = &info_d.as<BuildInfo<D>>()->n_valid;
 
   const int64_t B = t->leaf;
-  t->pts.reset((size_t)std::max<int64_t>(nv, 1) * sizeof(PointRec<D, DIM>));
-  t->orig.reset((size_t)std::max<int64_t>(nv, 1) * 4);
+  t->pts.reset((size_t)std::max<int64_t>(n, 1) * sizeof(PointRec<D, DIM>));
+  t->orig.reset((size_t)std::max<int64_t>(n, 1) * 4);
   t->fwd.reset((size_t)std::max<int64_t>(n, 1) * 4);
-  if (n > 0) PK_HIP(hipMemsetAsync(t->fwd.get(), 0xff, (size_t)n * 4, s));
 
-  // A binary tree whose leaves are non-empty has at most 2*nv - 1 nodes.
-  const int64_t cap = std::max<int64_t>(1, 2 * nv - 1);
+  // A binary tree whose leaves are non-empty has at most 2*nv - 1 <= 2*n - 1 nodes.
+  const int64_t cap = std::max<int64_t>(1, 2 * n - 1);
   DevBuf nodes((size_t)cap * sizeof(NodeRec));
   // Level table: every split launch reads its range from device memory and
   // publishes the next level, so the host reads it back only once per batch
@@ -356,47 +379,51 @@ void build_impl(pptk_tree* t, const T* dpts, int64_t n, hipStream_t s) {
   LevelState* state_d = (LevelState*)((char*)lev.get() + st_off);
   // root node, state and level table set by one kernel: no host copies
   level_init_kernel<<<1, kLevelBlock, 0, s>>>(nodes.as<NodeRec>(), state_d, lev_s, lev_e,
-                                               done.as<uint32_t>(), max_levels, (int32_t)nv);
+                                               done.as<uint32_t>(), max_levels, 0, nv_dev);
   PK_LAUNCH_CHECK();
   DevBuf keys, vals;
-  if (nv > 0) {
+  if (n > 0) {
     // K1: compaction + Morton codes
-    keys.reset((size_t)nv * 8);
-    vals.reset((size_t)nv * 4);
+    keys.reset((size_t)n * 8);
+    vals.reset((size_t)n * 4);
     // Key bits per axis: enough for ~2^kMortonExtraBits more cells per axis
-    // than the points need on a surface (ceil(log2 nv / DIM) + extra), at most
+    // than the points need on a surface (ceil(log2 n / DIM) + extra), at most
     // the full code; the low bits are dropped, so fewer radix passes.
     int lg = 0;
-    while ((int64_t(1) << lg) < nv) ++lg;
+    while ((int64_t(1) << lg) < n) ++lg;
     const int kbits = std::min(MortonBits<DIM>::value, (lg + DIM - 1) / DIM + kMortonExtraBits);
     const int kshift = DIM * (MortonBits<DIM>::value - kbits);
+    const uint64_t sentinel = (DIM * kbits >= 64) ? ~0ull : ((1ull << (DIM * kbits)) - 1ull);
     morton_kernel<T, D, DIM><<<(unsigned)nblk, kBlock, 0, s>>>(
         dpts, n, blk_off.as<uint32_t>(), info_d.as<BuildInfo<D>>(), keys.as<uint64_t>(),
-        vals.as<uint32_t>(), kshift);
+        vals.as<uint32_t>(), kshift, sentinel);
     PK_LAUNCH_CHECK();
-    // K2: sort
-    radix_sort_pairs(keys.as<uint64_t>(), vals.as<uint32_t>(), nv, DIM * kbits, s);
+    // K2: sort (stable: the invalid rows, placed behind the valid ones, stay
+    // behind any valid key equal to the sentinel)
+    radix_sort_pairs(keys.as<uint64_t>(), vals.as<uint32_t>(), n, DIM * kbits, s);
     // K3: gather in tree order
-    gather_kernel<T, D, DIM><<<grid_for(nv, kBlock), kBlock, 0, s>>>(
-        dpts, vals.as<uint32_t>(), nv, t->pts.as<PointRec<D, DIM>>(), t->orig.as<int32_t>(),
-        t->fwd.as<int32_t>());
+    gather_kernel<T, D, DIM><<<grid_for(n, kBlock), kBlock, 0, s>>>(
+        dpts, vals.as<uint32_t>(), n, nv_dev, t->pts.as<PointRec<D, DIM>>(),
+        t->orig.as<int32_t>(), t->fwd.as<int32_t>());
     PK_LAUNCH_CHECK();
   }
   // K3: split level by level until a level creates no children.
   const unsigned split_grid = (unsigned)std::min<int64_t>(
-      1024, std::max<int64_t>(kSplitProbeLanes / kLevelBlock, ceil_div(nv, (int64_t)B * kBlock)));
+      1024, std::max<int64_t>(kSplitProbeLanes / kLevelBlock, ceil_div(n, (int64_t)B * kBlock)));
   int nlev = 1;  // levels [0, nlev) hold nodes
   // the level table and state come back in one copy per batch of levels
   std::vector<char> tab(st_off + sizeof(LevelState));
   const int32_t* hs = (const int32_t*)tab.data();
   const int32_t* he = hs + (max_levels + 1);
+  BuildInfo<D> info;
   auto read_table = [&] {
     PK_HIP(hipMemcpyAsync(tab.data(), lev.get(), tab.size(), hipMemcpyDeviceToHost, s));
+    PK_HIP(hipMemcpyAsync(&info, info_d.get(), sizeof(info), hipMemcpyDeviceToHost, s));
     PK_HIP(hipStreamSynchronize(s));
   };
-  if (nv > 0) {
+  if (n > 0) {
     int balanced = 0;
-    while (((int64_t)B << balanced) < nv) ++balanced;
+    while (((int64_t)B << balanced) < n) ++balanced;
     for (int l0 = 0;;) {
       const int l1 = std::min(l0 == 0 ? std::max(kLevelBatch, balanced + 20) : l0 + kLevelBatch,
                               max_levels);
@@ -412,6 +439,14 @@ void build_impl(pptk_tree* t, const T* dpts, int64_t n, hipStream_t s) {
     }
   } else {
     read_table();
+  }
+  const int64_t nv = info.n_valid;
+  t->n_valid = nv;
+  for (int d = 0; d < 4; ++d) {
+    t->lo[d] = d < DIM ? (double)info.lo[d] : 0.0;
+    t->hi[d] = d < DIM ? (double)info.hi[d] : 0.0;
+    t->mlo[d] = (double)info.lo[d];
+    t->mscale[d] = (double)info.scale[d];
   }
   LevelState st;
   std::memcpy(&st, tab.data() + st_off, sizeof(st));

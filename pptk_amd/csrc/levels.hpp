@@ -33,7 +33,10 @@ static __global__ __launch_bounds__(kLevelBlock) void level_init_kernel(NodeRec*
                                                             int32_t* __restrict__ lev_s,
                                                             int32_t* __restrict__ lev_e,
                                                             uint32_t* __restrict__ done,
-                                                            int nlev, int32_t n) {
+                                                            int nlev, int32_t n_arg,
+                                                            const uint32_t* __restrict__ n_dev) {
+  // the root's size: n_arg, or read here when given on the device
+  const int32_t n = n_dev ? (int32_t)*n_dev : n_arg;
   for (int i = threadIdx.x; i <= nlev; i += kLevelBlock) {
     lev_s[i] = 0;
     lev_e[i] = i == 0 ? 1 : 0;

@@ -472,62 +472,139 @@ __global__ __launch_bounds__(kBlock) void rnear_packet_kernel(TreeView<D, DIM> t
     if constexpr (EXCL) size -= ((excl >= nr.a) & (excl < nr.b)) ? 1 : 0;
     return size;
   };
-  // Fill: copies the range of a node into the list of every lane of insm,
-  // once per 64-point chunk.
+  // Lane L's next output slot and list end as scalars.
+  auto write_window = [&](int L, int64_t& wL, int64_t& eL) {
+    const int64_t wv = ww[2 * L];
+    const int64_t ev_end = ww[2 * L + 1];
+    wL = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int32_t)(wv >> 32)) << 32) |
+                   (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)wv));
+    eL = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int32_t)(ev_end >> 32)) << 32) |
+                   (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)ev_end));
+  };
+  // Fill: copies the range of a node into the list of every lane of insm.
+  // The range's indices are held in registers kCopyGroup 64-point chunks at
+  // a time; per lane of insm and chunk, the slots are a scalar base plus
+  // the lane index, so a chunk is one compare and one store.
   auto copy_range = [&](const NodeRec& nr, uint64_t insm) {
-    for (int32_t s0 = nr.a; s0 < nr.b; s0 += 64) {
-      const int32_t p = s0 + lane;
-      const bool valid = p < nr.b;
-      const int32_t og = valid ? tv.orig[p] : 0;
+    constexpr int kCopyGroup = 4;
+    for (int32_t g0 = nr.a; g0 < nr.b; g0 += 64 * kCopyGroup) {
+      int32_t og[kCopyGroup];
+#pragma unroll
+      for (int c = 0; c < kCopyGroup; ++c) {
+        const int32_t p = g0 + 64 * c + lane;
+        og[c] = p < nr.b ? tv.orig[p] : 0;
+      }
       uint64_t lm = insm;
       while (lm) {
         const int L = __builtin_ctzll(lm);
         lm &= lm - 1ull;
-        const int64_t wL = ww[2 * L];
-        const int64_t eL = ww[2 * L + 1];
-        int64_t o = wL + (p - nr.a);
-        bool wr = valid;
+        int64_t wL, eL;
+        write_window(L, wL, eL);
+        // free slots of lane L's list (at most 2^31 - 1)
+        const int64_t fr = eL - wL;
+        const int32_t roomL = fr <= 0 ? 0 : (fr >= INT32_MAX ? INT32_MAX : (int32_t)fr);
+        int32_t exL = -1;
+        bool exin = false;
         if constexpr (EXCL) {
-          const int32_t exL = wq[L].orig;
-          // the own position is skipped: later points move down one slot
-          o -= ((exL >= nr.a) & (p > exL)) ? 1 : 0;
-          wr &= p != exL;
+          exL = __builtin_amdgcn_readfirstlane(wq[L].orig);
+          exin = (exL >= nr.a) & (exL < nr.b);
         }
-        if (wr & (o < eL)) out[o] = og;
+#pragma unroll
+        for (int c = 0; c < kCopyGroup; ++c) {
+          const int32_t s0 = g0 + 64 * c;
+          if (s0 >= nr.b) break;
+          // list slot of point p = s0 + lane: wL + off + lane, where off is
+          // s0 - nr.a, one less past the own position, which is skipped
+          int32_t off = s0 - nr.a;
+          int32_t d = 64;  // the own position's lane in this chunk
+          if constexpr (EXCL) {
+            if (exin) {
+              d = exL - s0;
+              if (d < 0) off -= 1;
+            }
+          }
+          // a lane writes when its point is inside the range (lane < rngc)
+          // and its slot inside the list (slot - wL - off < roomc)
+          const int32_t rm = roomL - off;
+          const int32_t rngc = nr.b - s0 < 64 ? nr.b - s0 : 64;
+          const int32_t roomc = rm <= 0 ? 0 : (rm < 64 ? rm : 64);
+          int32_t* dst = out + (wL + off);
+          if constexpr (EXCL) {
+            if ((uint32_t)d < 64u) {
+              // lanes past the own position move down one slot
+              const int32_t rel = lane - (lane > d ? 1 : 0);
+              if ((lane < rngc) & (lane != d) & (rel < roomc)) dst[(uint32_t)rel] = og[c];
+              continue;
+            }
+          }
+          const int32_t lim = rngc < roomc ? rngc : roomc;
+          if (lane < lim) dst[(uint32_t)lane] = og[c];
+        }
       }
     }
   };
   // Tests a leaf's points against each lane of strm: counts the hits, or
-  // writes them (fill).
+  // writes them (fill). The points are held in registers, kGroup 64-point
+  // chunks at a time (a whole leaf at bucket sizes up to 64 * kGroup), so a
+  // straddling lane's query, write pointer and list end are read once per
+  // group; the pointer and end are then scalars, and a hit is one store
+  // through a scalar base at the hit's rank.
   auto scan_leaf = [&](const NodeRec& nr, uint64_t strm) {
-    for (int32_t s0 = nr.a; s0 < nr.b; s0 += 64) {
-      const int32_t p = s0 + lane;
-      const bool valid = p < nr.b;
-      PointRec<D, DIM> rec{};
-      if (valid) rec = tv.pts[p];
-      // hit masks are ballots of the bare distance test, combined with
-      // the chunk's valid lanes (and the own position) as scalar masks
-      const uint64_t vmask = __ballot(valid);
+    constexpr int kGroup = sizeof(PointRec<D, DIM>) <= 16 ? 4 : 2;
+    for (int32_t g0 = nr.a; g0 < nr.b; g0 += 64 * kGroup) {
+      PointRec<D, DIM> rec[kGroup];
+      uint64_t vm[kGroup];
+#pragma unroll
+      for (int c = 0; c < kGroup; ++c) {
+        const int32_t p = g0 + 64 * c + lane;
+        const bool valid = p < nr.b;
+        rec[c] = PointRec<D, DIM>{};
+        if (valid) rec[c] = tv.pts[p];
+        vm[c] = __ballot(valid);
+      }
       uint64_t sm = strm;
       while (sm) {
         const int L = __builtin_ctzll(sm);
         sm &= sm - 1ull;
         const PointRec<D, DIM> qL = wq[L];
-        const bool in = dist2<D, DIM>(qL.c, rec.c) < r2;
-        uint64_t hits = __ballot(in) & vmask;
         int32_t exL = -1;
-        if constexpr (EXCL) {
-          exL = __builtin_amdgcn_readfirstlane(qL.orig);
-          const uint32_t dl = (uint32_t)(exL - s0);
-          hits &= dl < 64u ? ~(1ull << dl) : ~0ull;
+        if constexpr (EXCL) exL = __builtin_amdgcn_readfirstlane(qL.orig);
+        int32_t* dst = out;
+        uint32_t room = 0;  // free slots left in lane L's list, at most 2^31
+        int64_t wL = 0;
+        if constexpr (FILL) {
+          int64_t eL;
+          write_window(L, wL, eL);
+          const int64_t free_slots = eL - wL;
+          room = free_slots <= 0 ? 0u : (free_slots >= (1ll << 31) ? (1u << 31) : (uint32_t)free_slots);
+          dst = out + wL;
+        }
+        int32_t hl = 0;
+#pragma unroll
+        for (int c = 0; c < kGroup; ++c) {
+          if (vm[c] == 0ull) break;
+          // hit masks are ballots of the bare distance test, combined with
+          // the chunk's valid lanes (and the own position) as scalar masks
+          const bool in = dist2<D, DIM>(qL.c, rec[c].c) < r2;
+          uint64_t hits = __ballot(in) & vm[c];
+          if constexpr (EXCL) {
+            const uint32_t dl = (uint32_t)(exL - (g0 + 64 * c));
+            hits &= dl < 64u ? ~(1ull << dl) : ~0ull;
+          }
+          const uint32_t nh = (uint32_t)__popcll(hits);
+          if constexpr (FILL) {
+            const uint32_t rk = (uint32_t)rank_below(hits);
+            if (__builtin_amdgcn_inverse_ballot_w64(hits) & (rk < room)) dst[rk] = rec[c].orig;
+            const uint32_t adv = nh < room ? nh : room;
+            dst += adv;
+            room -= adv;
+          }
+          hl += (int32_t)nh;
         }
         if constexpr (FILL) {
-          const int64_t wL = ww[2 * L];
-          const int64_t o = wL + rank_below(hits);
-          if (in & valid & (p != exL) & (o < ww[2 * L + 1])) out[o] = rec.orig;
-          if (lane == 0) ww[2 * L] = wL + __popcll(hits);
+          if (lane == 0) ww[2 * L] = wL + hl;
         } else {
-          cnt += lane == L ? (int32_t)__popcll(hits) : 0;
+          cnt += lane == L ? hl : 0;
         }
       }
     }

@@ -38,16 +38,23 @@ def assert_knn_equal(lst, want, wcnt):
 def test_sort_pairs_matches_stable_argsort(gpu):
     from pptk_amd import capi
     rng = np.random.default_rng(5)
-    for n, bits in [(1, 63), (1000, 63), (4097, 20), (300_000, 63), (1_000_000, 32)]:
+    # (n, key bits, sorted bits): even and odd pass counts, one tile and
+    # several, more tiles than upsweep blocks, keys sharing their top digits;
+    # back-to-back calls reuse pool memory holding the last call's tile
+    # status words and counters
+    cases = [(1, 63, 64), (1000, 63, 64), (4097, 20, 20), (300_000, 63, 64),
+             (1_000_000, 32, 32), (1_000_000, 45, 45), (3_000_000, 41, 41),
+             (2_500_000, 24, 24), (2_500_000, 24, 24), (65_536, 7, 8)]
+    for n, bits, end_bit in cases:
         keys = rng.integers(0, 2 ** min(bits, 63), n, dtype=np.uint64)
         keys[: n // 3] = keys[n // 2]  # duplicates exercise stability
         vals = np.arange(n, dtype=np.uint32)
         k2, v2 = keys.copy(), vals.copy()
-        capi.check(capi.lib().pptk_sort_pairs_u64(capi._p(k2), capi._p(v2), n, 64,
+        capi.check(capi.lib().pptk_sort_pairs_u64(capi._p(k2), capi._p(v2), n, end_bit,
                                                   capi.MEM_HOST))
         order = np.argsort(keys, kind="stable")
-        assert np.array_equal(k2, keys[order]), n
-        assert np.array_equal(v2, vals[order]), n
+        assert np.array_equal(k2, keys[order]), (n, end_bit)
+        assert np.array_equal(v2, vals[order]), (n, end_bit)
 
 
 def test_exclusive_scan(gpu):

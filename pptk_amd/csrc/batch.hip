@@ -524,7 +524,7 @@ void enqueue_run(pptk_batch* b, const D* pts, int32_t* out_idx, int32_t* out_cnt
     lc();
     radix_sort_pairs_async(b->keys.as<uint64_t>(), b->vals.as<uint32_t>(), b->N, b->end_bit, b->ws,
                            s);
-    launches += 5 * ((b->end_bit + 7) / 8);
+    launches += 4 * ((b->end_bit + 7) / 8);  // histogram, two scan launches, scatter
     batch_gather_kernel<D, DIM><<<grid_of(b->N, kBlock), kBlock, 0, s>>>(
         pts, b->keys.as<uint64_t>(), b->vals.as<uint32_t>(), b->N, b->shift,
         b->coff.as<int64_t>(), b->vcnt.as<uint32_t>(), b->C, b->rec.as<PointRec<D, DIM>>(),

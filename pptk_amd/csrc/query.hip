@@ -880,9 +880,11 @@ __global__ __launch_bounds__(kBlock) void query_position_kernel(const int32_t* _
 constexpr int64_t kOrderMin = 4096;
 
 // Point queries are ordered on the top kQueryOrderBits bits of their Morton
-// code (16 bits per axis in 3-D): finer cells than any wave's extent, in
-// fewer radix passes than the full code.
-constexpr int kQueryOrderBits = 48;
+// code (13 to 14 bits per axis in 3-D): finer cells than any wave's extent,
+// in 5 radix passes instead of the full code's 8. On the 10M-point lidar
+// scene with 1M queries: count 1.41 / 1.39 / 1.38 / 1.58 ms at 48 / 40 / 36
+// / 32 bits, the fill 2.06 ms at 48 to 40 bits and 2.08 at 36.
+constexpr int kQueryOrderBits = 40;
 
 inline size_t align256(size_t b) { return (b + 255) / 256 * 256; }
 

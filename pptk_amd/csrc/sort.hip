@@ -4,8 +4,9 @@
 // Radix sort, one 8-bit digit per pass:
 //   1. radix_hist_kernel    per-tile digit histogram, stored digit-major
 //                           (hist[digit * tiles + tile]);
-//   2. exclusive scan of the histogram (scan_* kernels below) gives every
-//      (digit, tile) its global output offset;
+//   2. exclusive scan of the histogram (scan_reduce_kernel: per-chunk sums;
+//      scan_down_kernel: each chunk's scan plus the sum of the chunks
+//      before it) gives every (digit, tile) its global output offset;
 //   3. radix_scatter_kernel ranks keys stably inside the tile with wave
 //      ballots (peer mask of equal digits) and per-wave LDS counters, stages
 //      the tile in LDS in digit order, and writes each digit's run of the
@@ -255,12 +256,32 @@ __global__ __launch_bounds__(kScanBlock) void scan_single_kernel(T* __restrict__
   if (threadIdx.x == 0) data[n] = carry;
 }
 
-// Exclusive scan of each tile plus its tile offset. Each thread owns 16
-// consecutive elements of the tile.
+// Sum of one value per thread over the block (kScanBlock threads).
+template <typename T>
+__device__ __forceinline__ T block_sum(T v) {
+  __shared__ T ws[kScanBlock / 64];
+  v = wave_sum(v);
+  if (lane_id() == 0) ws[threadIdx.x >> 6] = v;
+  __syncthreads();
+  T t = 0;
+#pragma unroll
+  for (int i = 0; i < kScanBlock / 64; ++i) t += ws[i];
+  __syncthreads();
+  return t;
+}
+
+// Exclusive scan of each tile plus its tile offset, which the block sums
+// from the earlier tiles' sums itself (no separate pass over the sums).
+// Each thread owns 16 consecutive elements of the tile. With `total`, the
+// last block writes the sum of all n values there.
 template <typename Tin, typename Tout>
 __global__ __launch_bounds__(kScanBlock) void scan_down_kernel(const Tin* __restrict__ in,
                                                                Tout* __restrict__ out, int64_t n,
-                                                               const Tout* __restrict__ tile_off) {
+                                                               const Tout* __restrict__ tile_sums,
+                                                               Tout* __restrict__ total) {
+  Tout before = 0;
+  for (int64_t j = threadIdx.x; j < (int64_t)blockIdx.x; j += kScanBlock) before += tile_sums[j];
+  const Tout tile_off = block_sum(before);
   const int64_t i0 = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
   Tout v[kScanItems];
   Tout local = 0;
@@ -269,13 +290,14 @@ __global__ __launch_bounds__(kScanBlock) void scan_down_kernel(const Tin* __rest
     v[e] = (i0 + e < n) ? (Tout)in[i0 + e] : (Tout)0;
     local += v[e];
   }
-  Tout total;
-  Tout run = tile_off[blockIdx.x] + block_excl_scan(local, &total);
+  Tout tile_total;
+  Tout run = tile_off + block_excl_scan(local, &tile_total);
 #pragma unroll
   for (int e = 0; e < kScanItems; ++e) {
     if (i0 + e < n) out[i0 + e] = run;
     run += v[e];
   }
+  if (total && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *total = tile_off + tile_total;
 }
 
 }  // namespace
@@ -288,15 +310,13 @@ void exclusive_scan(const Tin* in, Tout* out, int64_t n, hipStream_t s) {
     return;
   }
   const int64_t tiles = ceil_div(n, kScanTile);
-  DevBuf sums((size_t)(tiles + 1) * sizeof(Tout));
+  DevBuf sums((size_t)tiles * sizeof(Tout));
   Tout* ts = sums.as<Tout>();
   scan_reduce_kernel<Tin, Tout><<<dim3((unsigned)tiles), dim3(kScanBlock), 0, s>>>(in, n, ts);
   PK_LAUNCH_CHECK();
-  scan_single_kernel<Tout><<<1, kScanBlock, 0, s>>>(ts, tiles);
+  scan_down_kernel<Tin, Tout><<<dim3((unsigned)tiles), dim3(kScanBlock), 0, s>>>(in, out, n, ts,
+                                                                                 out + n);
   PK_LAUNCH_CHECK();
-  scan_down_kernel<Tin, Tout><<<dim3((unsigned)tiles), dim3(kScanBlock), 0, s>>>(in, out, n, ts);
-  PK_LAUNCH_CHECK();
-  PK_HIP(hipMemcpyAsync(out + n, ts + tiles, sizeof(Tout), hipMemcpyDeviceToDevice, s));
   // `sums` returns to the pool fenced on s (no other stream gets it before
   // this work is done), so no synchronisation here
 }
@@ -356,10 +376,8 @@ void radix_sort_pairs_async(uint64_t* keys, uint32_t* vals, int64_t n, int end_b
     scan_reduce_kernel<uint32_t, uint32_t>
         <<<dim3((unsigned)scan_tiles), dim3(kScanBlock), 0, s>>>(hist, hlen, ts);
     PK_LAUNCH_CHECK();
-    scan_single_kernel<uint32_t><<<1, kScanBlock, 0, s>>>(ts, scan_tiles);
-    PK_LAUNCH_CHECK();
     scan_down_kernel<uint32_t, uint32_t><<<dim3((unsigned)scan_tiles), dim3(kScanBlock), 0, s>>>(
-        hist, offs, hlen, ts);
+        hist, offs, hlen, ts, nullptr);
     PK_LAUNCH_CHECK();
     radix_scatter_kernel<<<dim3((unsigned)tiles), dim3(kBlock), 0, s>>>(k0, v0, k1, v1, n, shift,
                                                                         offs, (int)tiles);

@@ -28,7 +28,7 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-REPORT_DIR = os.environ.get("PPTK_REPORT_DIR", "gpurun_out")
+REPORT_DIR = os.environ.get("PPTK_REPORT_DIR", "reports")
 N, M, K, DMAX = 10_000_000, 1_000_000, 16, 0.1
 
 

@@ -39,9 +39,8 @@ def test_sort_pairs_matches_stable_argsort(gpu):
     from pptk_amd import capi
     rng = np.random.default_rng(5)
     # (n, key bits, sorted bits): even and odd pass counts, one tile and
-    # several, more tiles than upsweep blocks, keys sharing their top digits;
-    # back-to-back calls reuse pool memory holding the last call's tile
-    # status words and counters
+    # hundreds, keys sharing their top digits; back-to-back calls reuse pool
+    # memory holding the last call's histograms and offsets
     cases = [(1, 63, 64), (1000, 63, 64), (4097, 20, 20), (300_000, 63, 64),
              (1_000_000, 32, 32), (1_000_000, 45, 45), (3_000_000, 41, 41),
              (2_500_000, 24, 24), (2_500_000, 24, 24), (65_536, 7, 8)]
@@ -60,7 +59,8 @@ def test_sort_pairs_matches_stable_argsort(gpu):
 def test_exclusive_scan(gpu):
     from pptk_amd import capi
     rng = np.random.default_rng(6)
-    for n in [0, 1, 4095, 4096, 4097, 1_000_003]:
+    # up to 733 chunks of 4096: more chunk sums than a scan block has threads
+    for n in [0, 1, 4095, 4096, 4097, 1_000_003, 1_100_000, 3_000_017]:
         x = rng.integers(0, 1000, n).astype(np.int32)
         out = np.empty(n + 1, np.int64)
         capi.check(capi.lib().pptk_exclusive_scan_i32(capi._p(x), capi._p(out), n,
@@ -175,7 +175,7 @@ def test_normals_c0_float64_every_row_vs_fp64(gpu):
     report = {"workload": "uniform cube 100000 float64 points (seed 0), normals k=16 r=inf",
               "rows": 100_000, "angle_deg": q,
               "fraction_eigengap_lt_1e-3": float(np.mean(gap < 1e-3))}
-    d = os.environ.get("PPTK_REPORT_DIR", "gpurun_out")
+    d = os.environ.get("PPTK_REPORT_DIR", "reports")
     if os.path.isdir(d):
         with open(os.path.join(d, "c0_normals.json"), "w") as f:
             json.dump(report, f, indent=1)

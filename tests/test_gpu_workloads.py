@@ -22,7 +22,7 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-REPORT_DIR = os.environ.get("PPTK_REPORT_DIR", "gpurun_out")
+REPORT_DIR = os.environ.get("PPTK_REPORT_DIR", "reports")
 
 
 def _report(name, obj):

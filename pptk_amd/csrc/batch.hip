@@ -639,8 +639,9 @@ pptk_batch* check_batch(const pptk_batch* b) {
 }
 
 void drop_graph(pptk_batch* b) {
-  if (b->exec) hipGraphExecDestroy(b->exec);
-  if (b->graph) hipGraphDestroy(b->graph);
+  // teardown: a failure to free leaves nothing to recover
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
   b->exec = nullptr;
   b->graph = nullptr;
   for (auto& k : b->gkey) k = nullptr;
@@ -811,9 +812,11 @@ pptk_status api_batch_run(pptk_batch* bp, const void* pts, int32_t* out_idx, int
         try {
           enqueue();
         } catch (...) {
+          // end the capture before rethrowing; its own errors would hide
+          // the one being reported
           hipGraph_t g = nullptr;
-          hipStreamEndCapture(s, &g);
-          if (g) hipGraphDestroy(g);
+          (void)hipStreamEndCapture(s, &g);
+          if (g) (void)hipGraphDestroy(g);
           throw;
         }
         PK_HIP(hipStreamEndCapture(s, &b->graph));
@@ -887,7 +890,7 @@ void api_batch_free(pptk_batch* b) {
   }
   drop_graph(b);
   for (auto& e : b->ev)
-    if (e) hipEventDestroy(e);
+    if (e) (void)hipEventDestroy(e);
   b->magic = 0;
   delete b;
 }

@@ -234,9 +234,11 @@ def pair_dist2(q, p, dtype):
     q = np.asarray(q, dtype)
     p = np.asarray(p, dtype)
     s = np.zeros(np.broadcast_shapes(q.shape[:-1], p.shape[:-1]), dtype)
-    for i in range(q.shape[-1]):
-        t = p[..., i] - q[..., i]
-        s = s + t * t
+    # non-finite rows are inputs here: their inf/NaN distances are expected
+    with np.errstate(over="ignore", invalid="ignore"):
+        for i in range(q.shape[-1]):
+            t = p[..., i] - q[..., i]
+            s = s + t * t
     return s
 
 

@@ -734,89 +734,107 @@ __global__ __launch_bounds__(kBlock) void rnear_packet_kernel(TreeView<D, DIM> t
   }
 }
 
-// Cost key of each r-near fill wave (pk consecutive queries): the
-// complement of its output count. Thread t serves row order[t] (or t); for
-// self queries of every point (row_of_pos set) it serves row_of_pos[t].
-__global__ __launch_bounds__(kBlock) void rnear_wave_cost_kernel(const int32_t* __restrict__ order,
-                                                                 const int32_t* __restrict__ row_of_pos,
-                                                                 const int64_t* __restrict__ offsets,
-                                                                 int64_t m, int64_t nrows, int pk,
-                                                                 uint64_t* __restrict__ keys) {
-  const int64_t w = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (w * pk >= m) return;
-  int64_t c = 0;
-  for (int l = 0; l < pk; ++l) {
-    const int64_t t = w * pk + l;
-    if (t >= m) break;
-    const int64_t row = row_of_pos ? row_of_pos[t] : (order ? order[t] : t);
-    if (row >= 0 && row < nrows) {
-      const int64_t n = offsets[row + 1] - offsets[row];
-      c += n > 0 ? n : 0;
-    }
-  }
-  keys[w] = 0xffffffffull - (uint64_t)(c < 0xffffffffll ? c : 0xffffffffll);
+// Launch order of the fill's waves, heaviest first: waves bucketed by
+// floor(log2(output + 1)) of their output count from the count pass, buckets
+// in decreasing order, waves in index order inside a bucket. Three launches,
+// one thread per wave in the first and last:
+//   rnear_wave_bucket_count_kernel  each wave's bucket, each block's count
+//                                   per bucket;
+//   scan_u32_inplace_async          one block: every (bucket, block) run's
+//                                   start, buckets heaviest first;
+//   rnear_wave_order_kernel         each wave's slot: its run's start plus
+//                                   its rank among the block's waves of its
+//                                   bucket (ballots).
+// (One 1,024-thread block for all of it took 61 us on 62.5K waves, plus a
+// 17 us cost-key kernel; a 32-bit radix sort of the keys before that took
+// 4 passes, 20 launches.)
+constexpr int kBucketBlock = 256;
+constexpr int kBucketWaves = kBucketBlock / 64;
+constexpr int kBuckets = 33;
+
+__device__ __forceinline__ int wave_bucket(int64_t c) {  // 0 .. 32
+  const uint64_t v = (uint64_t)(c < 0xffffffffll ? c : 0xffffffffll);
+  return 63 - __clzll(v + 1ull);
 }
 
-// Launch order of the fill's waves, heaviest first: waves bucketed by
-// floor(log2(output + 1)) from their rnear_wave_cost_kernel keys, buckets in
-// decreasing order, waves in index order inside a bucket. One block; each
-// wave of the block ranks a contiguous span of keys with ballots (no atomics).
-// One launch in place of a 32-bit radix sort of the keys (4 passes, 20
-// launches).
-constexpr int kBucketBlock = 1024;
-constexpr int kBuckets = 33;
-__global__ __launch_bounds__(kBucketBlock) void rnear_wave_bucket_kernel(
-    const uint64_t* __restrict__ keys, int64_t nw, uint32_t* __restrict__ worder) {
-  constexpr int kWaves = kBucketBlock / 64;
-  __shared__ uint32_t cnt[kWaves][kBuckets];
+// Calls f(k, m) for each bucket k present among the wave's lanes (b = -1: no
+// fill wave), m the lanes holding k; one ballot per bucket present.
+template <typename F>
+__device__ __forceinline__ void each_bucket(int b, F f) {
+  uint64_t todo = __ballot(b >= 0);
+  while (todo) {
+    const int k = __builtin_amdgcn_readlane(b, __builtin_ctzll(todo));
+    const uint64_t m = __ballot(b == k);
+    todo &= ~m;
+    f(k, m);
+  }
+}
+
+// Fill wave w serves rows order[w*pk .. w*pk+pk) (pk <= kRnearMaxFillWidth),
+// or those positions; for self queries of every point (row_of_pos set)
+// row_of_pos[t]. Writes each wave's bucket and the block's count of bucket
+// k at cnt[(kBuckets - 1 - k) * nblk + block] (runs in slot order).
+constexpr int kRnearMaxFillWidth = 16;
+__global__ __launch_bounds__(kBucketBlock) void rnear_wave_bucket_count_kernel(
+    const int32_t* __restrict__ order, const int32_t* __restrict__ row_of_pos,
+    const int64_t* __restrict__ offsets, int64_t m, int64_t nrows, int pk, int64_t nw,
+    uint8_t* __restrict__ bucket_of, uint32_t* __restrict__ cnt) {
+  __shared__ uint32_t bc[kBuckets];
+  const int lane = threadIdx.x & 63;
+  if (threadIdx.x < kBuckets) bc[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t w = (int64_t)blockIdx.x * kBucketBlock + threadIdx.x;
+  int b = -1;
+  if (w < nw) {
+    // the wave's rows first, then all their offsets: every read in flight
+    int64_t row[kRnearMaxFillWidth];
+#pragma unroll
+    for (int l = 0; l < kRnearMaxFillWidth; ++l) {
+      const int64_t t = w * pk + l;
+      row[l] = -1;
+      if (l < pk && t < m) row[l] = row_of_pos ? row_of_pos[t] : (order ? order[t] : t);
+    }
+    int64_t c = 0;
+#pragma unroll
+    for (int l = 0; l < kRnearMaxFillWidth; ++l) {
+      if (row[l] >= 0 && row[l] < nrows) {
+        const int64_t n = offsets[row[l] + 1] - offsets[row[l]];
+        c += n > 0 ? n : 0;
+      }
+    }
+    b = wave_bucket(c);
+    bucket_of[w] = (uint8_t)b;
+  }
+  each_bucket(b, [&](int k, uint64_t msk) {
+    if (lane == 0) atomicAdd(&bc[k], (uint32_t)__popcll(msk));
+  });
+  __syncthreads();
+  if (threadIdx.x < kBuckets)
+    cnt[(int64_t)(kBuckets - 1 - threadIdx.x) * gridDim.x + blockIdx.x] = bc[threadIdx.x];
+}
+
+__global__ __launch_bounds__(kBucketBlock) void rnear_wave_order_kernel(
+    const uint8_t* __restrict__ bucket_of, const uint32_t* __restrict__ start, int64_t nw,
+    uint32_t* __restrict__ worder) {
+  __shared__ uint32_t wc[kBucketWaves][kBuckets];  // each wave's count per bucket
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
-  const int64_t span = (nw + kWaves - 1) / kWaves;
-  const int64_t lo = wv * span;
-  const int64_t hi = lo + span < nw ? lo + span : nw;
-  auto bucket = [](uint64_t key) {
-    const uint64_t c = 0xffffffffull - key;  // the wave's output, < 2^32
-    return 63 - __clzll(c + 1ull);           // 0 .. 32
-  };
-  // the buckets present among 64 keys, one ballot each
-  auto each_bucket = [&](int b, auto f) {
-    uint64_t todo = __ballot(b >= 0);
-    while (todo) {
-      const int k = __builtin_amdgcn_readlane(b, __builtin_ctzll(todo));
-      const uint64_t m = __ballot(b == k);
-      todo &= ~m;
-      f(k, m);
-    }
-  };
-  uint32_t mine = 0;  // lane b counts bucket b of this span (kBuckets <= 64)
-  for (int64_t i0 = lo; i0 < hi; i0 += 64) {
-    const int64_t i = i0 + lane;
-    const int b = i < hi ? bucket(keys[i]) : -1;
-    each_bucket(b, [&](int k, uint64_t m) { mine += lane == k ? (uint32_t)__popcll(m) : 0u; });
-  }
-  if (lane < kBuckets) cnt[wv][lane] = mine;
+  const int64_t w = (int64_t)blockIdx.x * kBucketBlock + threadIdx.x;
+  const int b = w < nw ? (int)bucket_of[w] : -1;
+  uint32_t mine = 0;  // lane k: this wave's count of bucket k
+  each_bucket(b, [&](int k, uint64_t msk) { mine += lane == k ? (uint32_t)__popcll(msk) : 0u; });
+  if (lane < kBuckets) wc[wv][lane] = mine;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    // exclusive offsets: buckets from the heaviest down, spans in order
-    uint32_t run = 0;
-    for (int k = kBuckets - 1; k >= 0; --k)
-      for (int v = 0; v < kWaves; ++v) {
-        const uint32_t c = cnt[v][k];
-        cnt[v][k] = run;
-        run += c;
-      }
+  // lane k: the slot of this wave's first fill wave of bucket k
+  uint32_t base = 0;
+  if (lane < kBuckets) {
+    base = start[(int64_t)(kBuckets - 1 - lane) * gridDim.x + blockIdx.x];
+    for (int v = 0; v < wv; ++v) base += wc[v][lane];
   }
-  __syncthreads();
-  uint32_t base = lane < kBuckets ? cnt[wv][lane] : 0u;  // lane b: next slot of bucket b
-  for (int64_t i0 = lo; i0 < hi; i0 += 64) {
-    const int64_t i = i0 + lane;
-    const int b = i < hi ? bucket(keys[i]) : -1;
-    each_bucket(b, [&](int k, uint64_t m) {
-      const uint32_t bk = (uint32_t)__builtin_amdgcn_readlane((int)base, k);
-      if (b == k) worder[bk + rank_below(m)] = (uint32_t)i;
-      base += lane == k ? (uint32_t)__popcll(m) : 0u;
-    });
-  }
+  each_bucket(b, [&](int k, uint64_t msk) {
+    const uint32_t bk = (uint32_t)__builtin_amdgcn_readlane((int)base, k);
+    if (b == k) worder[bk + rank_below(msk)] = (uint32_t)w;
+  });
 }
 
 template <typename T>
@@ -832,6 +850,7 @@ inline unsigned grid_of(int64_t n) { return (unsigned)std::max<int64_t>(1, ceil_
 // count 64/32/16/8 -> 2.16/1.85/2.00/2.53 ms; fill 64/32/16/8 -> 4.19/3.44/3.37/4.03 ms.
 constexpr int kRnearCountWidth = 32;
 constexpr int kRnearFillWidth = 16;
+static_assert(kRnearFillWidth <= kRnearMaxFillWidth, "fill waves wider than the wave bucketing reads");
 
 }  // namespace
 
@@ -1332,14 +1351,20 @@ void rnear_fill_run(const pptk_tree* t, const pptk_queries* qs, double dmax, con
     // start are short ones
     const uint32_t* worder = nullptr;
     if (tv.nroots == 1 && nw > 1) {
-      wkeys.reset((size_t)nw * 8);
+      const int64_t nblk = ceil_div(nw, kBucketBlock);
+      // run lengths in slot order, scanned in place into run starts (+ total)
+      const size_t cbytes = (size_t)(nblk * kBuckets + 1) * 4;
+      wkeys.reset(align256((size_t)nw) + cbytes);
       wvals.reset((size_t)nw * 4);
-      rnear_wave_cost_kernel<<<grid_of(nw), kBlock, 0, s>>>(
+      uint8_t* bof = wkeys.as<uint8_t>();
+      uint32_t* bcnt = (uint32_t*)(bof + align256((size_t)nw));
+      rnear_wave_bucket_count_kernel<<<(unsigned)nblk, kBucketBlock, 0, s>>>(
           pq.order, pq.mode == kQueryAll ? tv.orig : nullptr, (const int64_t*)doff.ptr, pq.m,
-          nrows, pk, wkeys.as<uint64_t>());
+          nrows, pk, nw, bof, bcnt);
       PK_LAUNCH_CHECK();
-      rnear_wave_bucket_kernel<<<1, kBucketBlock, 0, s>>>(wkeys.as<uint64_t>(), nw,
-                                                           wvals.as<uint32_t>());
+      scan_u32_inplace_async(bcnt, nblk * kBuckets, s);
+      rnear_wave_order_kernel<<<(unsigned)nblk, kBucketBlock, 0, s>>>(bof, bcnt, nw,
+                                                                      wvals.as<uint32_t>());
       PK_LAUNCH_CHECK();
       worder = wvals.as<uint32_t>();
     }

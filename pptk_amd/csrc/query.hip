@@ -990,7 +990,7 @@ void order_queries(const pptk_tree* t, PreparedQueries& pq, hipStream_t s,
     else if (t->dim == 3) launch(std::integral_constant<int, 3>());
     else launch(std::integral_constant<int, 4>());
   }
-  radix_sort_pairs_async(keys, ord, m, bits, ws, s);
+  radix_sort_pairs_async(keys, ord, m, bits, ws, s, /*keys_out=*/false);
   pq.order = (const int32_t*)ord;
   if (scratch) return;  // stream order keeps the scratch valid for the queries
   // The cache entry is published only once the order is complete.

@@ -355,7 +355,7 @@ void SortWorkspace::reserve(int64_t n) {
 }
 
 void radix_sort_pairs_async(uint64_t* keys, uint32_t* vals, int64_t n, int end_bit,
-                            SortWorkspace& ws, hipStream_t s) {
+                            SortWorkspace& ws, hipStream_t s, bool keys_out) {
   if (n <= 1 || end_bit <= 0) return;
   if (n > (int64_t)0x7fffffff) fail(PPTK_E_VALUE, "radix sort: n=%lld too large", (long long)n);
   if (n > ws.n_cap) fail(PPTK_E_RUNTIME, "radix sort workspace too small");
@@ -386,7 +386,8 @@ void radix_sort_pairs_async(uint64_t* keys, uint32_t* vals, int64_t n, int end_b
     uint32_t* tv = v0; v0 = v1; v1 = tv;
   }
   if (passes & 1) {
-    PK_HIP(hipMemcpyAsync(keys, k0, (size_t)n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+    if (keys_out)
+      PK_HIP(hipMemcpyAsync(keys, k0, (size_t)n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
     PK_HIP(hipMemcpyAsync(vals, v0, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
   }
 }

@@ -27,9 +27,11 @@ struct SortWorkspace {
 };
 
 // As radix_sort_pairs, with no allocation and no synchronisation (safe to
-// record into a hip graph).
+// record into a hip graph). With keys_out false only the values are sorted
+// in place; the keys are left in an unspecified order (an odd number of
+// passes then copies one array back instead of two).
 void radix_sort_pairs_async(uint64_t* keys, uint32_t* vals, int64_t n, int end_bit,
-                            SortWorkspace& ws, hipStream_t s);
+                            SortWorkspace& ws, hipStream_t s, bool keys_out = true);
 
 // In-place exclusive scan of n uint32 values by one block; data[n] receives
 // the total. Asynchronous.

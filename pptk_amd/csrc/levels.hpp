@@ -180,13 +180,24 @@ static __global__ __launch_bounds__(kLevelBlock) void leaf_box_kernel(const Poin
       lo[d] = pos_inf<D>();
       hi[d] = -pos_inf<D>();
     }
-    for (int32_t p = nr.a + sub; p < nr.b; p += 8) {
-      const PointRec<D, DIM> r = rec[p];
-      leafof[p] = (int32_t)i;
+    // four records per lane in flight: a lane's loads do not wait on each
+    // other (the last ones of a leaf repeat its last point)
+    for (int32_t p0 = nr.a + sub; p0 < nr.b; p0 += 32) {
+      PointRec<D, DIM> r[4];
 #pragma unroll
-      for (int d = 0; d < DIM; ++d) {
-        lo[d] = r.c[d] < lo[d] ? r.c[d] : lo[d];
-        hi[d] = r.c[d] > hi[d] ? r.c[d] : hi[d];
+      for (int u = 0; u < 4; ++u) {
+        const int32_t p = p0 + 8 * u;
+        r[u] = rec[p < nr.b ? p : nr.b - 1];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int32_t p = p0 + 8 * u;
+        if (p < nr.b) leafof[p] = (int32_t)i;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+          lo[d] = r[u].c[d] < lo[d] ? r[u].c[d] : lo[d];
+          hi[d] = r[u].c[d] > hi[d] ? r[u].c[d] : hi[d];
+        }
       }
     }
 #pragma unroll

@@ -253,6 +253,26 @@ __device__ __forceinline__ D box_max_dist2(const BoxRec<D, DIM>& b, const D* q) 
   return s;
 }
 
+// box_min_dist2 for sgn = +1, box_max_dist2 for sgn = -1, bit for bit, in
+// the same instructions: sq = sgn * q (exact), so fma(sgn, lo, -sq) and
+// fma(-sgn, hi, sq) are sgn * (lo - q) and sgn * (q - hi), each one rounding
+// of the exact difference as the plain subtractions are. For sgn = -1 the
+// axis term is max(q - lo, hi - q)^2, which is max((lo - q)^2, (hi - q)^2)
+// because rounding a square is monotone in the magnitude. Lets half a wave
+// take the near bound and the other half the far bound of the same node.
+template <typename D, int DIM>
+__device__ __forceinline__ D box_near_far_dist2(const BoxRec<D, DIM>& b, const D* sq, D sgn) {
+  D s = D(0);
+#pragma unroll
+  for (int i = 0; i < DIM; ++i) {
+    const D x = fma(sgn, b.lo[i], -sq[i]);
+    const D y = fma(-sgn, b.hi[i], sq[i]);
+    const D e = fmax(fmax(x, y), D(0));
+    s = s + e * e;
+  }
+  return s;
+}
+
 // Workgroups of a launch are dispatched to the chip's 8 XCDs in turn
 // (workgroup i to XCD i mod 8), and each XCD has its own L2. xcd_block maps
 // the dispatched workgroup b of a grid of g to the index of the work it

@@ -369,8 +369,9 @@ struct RnearEventView {
 };
 
 // r-near on a single-root tree, one traversal per wave (count: FILL = false,
-// fill: FILL = true). A wave takes pk <= 64 consecutive queries (lanes pk..63
-// idle); consecutive queries are neighbours on the Morton curve, so their
+// fill: FILL = true). A wave takes pk <= 32 consecutive queries in lanes
+// 0 .. pk-1 (lanes 32 .. 32+pk-1 hold the same queries for the far-bound
+// half of each node test; the others idle); consecutive queries are neighbours on the Morton curve, so their
 // balls overlap: the wave walks the tree once, in left-to-right order, and
 // visits a node when any lane's ball reaches it. Fewer queries per wave
 // split the dense regions' work (thousands of hits per query) over more
@@ -423,16 +424,22 @@ __global__ __launch_bounds__(kBlock) void rnear_packet_kernel(TreeView<D, DIM> t
   const int lane = threadIdx.x & 63;
   int64_t w = xcd_block(blockIdx.x, gridDim.x, xrun) * (kBlock / 64) + (threadIdx.x >> 6);
   if (wave_order && w < nw) w = wave_order[w];
-  const int64_t t = w * pk + lane;
+  // Lanes 0 .. pk-1 serve the wave's queries (pk <= 32). Lane 32 + l holds
+  // lane l's query too: in the walk the low half tests each node's near
+  // bound and the high half its far bound, in one pass.
+  const int ql = lane & 31;
+  const int64_t t = w * pk + ql;
   D q[DIM];
 #pragma unroll
   for (int d = 0; d < DIM; ++d) q[d] = D(0);
   int32_t excl = -1;
   int64_t row = 0;
   bool ok = false, has_row = false;
-  if (lane < pk && t < m)
+  if (ql < pk && t < m)
     has_row = resolve_query<D, DIM>(tv, mode, qlist, qpts, order, t, nrows, q, excl, row, ok);
+  if (lane >= 32) has_row = false;
   const bool active = has_row && ok;
+  const D sgn = lane < 32 ? D(1) : D(-1);
   // Lane values read by the whole wave live in LDS: the query (and its own
   // position) and, for the fill, the next output slot and the list end; one
   // broadcast read fetches a lane's record. The write pointers are kept only
@@ -682,21 +689,23 @@ __global__ __launch_bounds__(kBlock) void rnear_packet_kernel(TreeView<D, DIM> t
     };
     int32_t done_end = INT32_MIN;  // nodes starting before this are taken
     int32_t node = actm ? 0 : -1;
+    D sq[DIM];  // sgn * q
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) sq[d] = sgn * q[d];
     while (node >= 0) {
       node = __builtin_amdgcn_readfirstlane(node);
       const NodeRec nr = tv.node[node];
       const BoxRec<D, DIM> bx = tv.box[node];
       const bool c_new = nr.a >= done_end;
-      const bool c_min = box_min_dist2<D, DIM>(bx, q) < r2;
-      const bool hit = active & c_new & c_min;
-      const uint64_t hitm = __ballot(c_new) & __ballot(c_min) & actm;
+      // low half: the ball reaches the box; high half: the ball holds it
+      const uint64_t bm = __ballot(box_near_far_dist2<D, DIM>(bx, sq, sgn) < r2);
+      const uint64_t hitm = __ballot(c_new) & bm & actm;
       bool descend = false;
       if (hitm) {
-        const bool c_max = box_max_dist2<D, DIM>(bx, q) < r2;
-        const bool inside = hit & c_max;
-        const uint64_t cmaxm = __ballot(c_max);
+        const uint64_t cmaxm = bm >> 32;
         const uint64_t insm = hitm & cmaxm;
         const uint64_t strm = hitm & ~cmaxm;
+        const bool inside = __builtin_amdgcn_inverse_ballot_w64(insm);
         if (inside) done_end = nr.b;
         if constexpr (FILL) {
           if (insm) copy_range(nr, insm);
@@ -851,6 +860,8 @@ inline unsigned grid_of(int64_t n) { return (unsigned)std::max<int64_t>(1, ceil_
 constexpr int kRnearCountWidth = 32;
 constexpr int kRnearFillWidth = 16;
 static_assert(kRnearFillWidth <= kRnearMaxFillWidth, "fill waves wider than the wave bucketing reads");
+static_assert(kRnearCountWidth <= 32 && kRnearFillWidth <= 32,
+              "r-near packets fill the low half of a wave (the high half mirrors it)");
 
 }  // namespace
 

@@ -296,16 +296,18 @@ __global__ __launch_bounds__(kBlock) void morton_kernel(const T* __restrict__ pt
   }
 }
 
-// Tree-order records and both row maps from the sorted order of all n rows:
-// positions [0, n_valid) are the valid rows in tree order, the rest the
-// invalid rows, whose forward entry is -1.
-template <typename T, typename D, int DIM>
-__global__ __launch_bounds__(kBlock) void gather_kernel(const T* __restrict__ pts,
-                                                        const uint32_t* __restrict__ order,
-                                                        int64_t n, const uint32_t* __restrict__ nv_dev,
-                                                        PointRec<D, DIM>* __restrict__ rec,
-                                                        int32_t* __restrict__ orig,
-                                                        int32_t* __restrict__ fwd) {
+// Tree order from the sorted order of all n rows, in two launches: the row
+// maps (positions [0, n_valid) are the valid rows in tree order, the rest the
+// invalid rows, whose forward entry is -1), then the records, each input row
+// read in input order and written to its position. Scattering the 16-byte
+// records costs less than gathering the 12-byte input rows in tree order
+// (10M-point lidar scene: build 1.645 -> 1.58 ms; the forward map's 4-byte
+// entries are scattered either way).
+__global__ __launch_bounds__(kBlock) void row_maps_kernel(const uint32_t* __restrict__ order,
+                                                          int64_t n,
+                                                          const uint32_t* __restrict__ nv_dev,
+                                                          int32_t* __restrict__ orig,
+                                                          int32_t* __restrict__ fwd) {
   const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (p >= n) return;
   const uint32_t i = order[p];
@@ -313,13 +315,23 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(const T* __restrict__ pt
     fwd[i] = -1;
     return;
   }
-  PointRec<D, DIM> r;
-#pragma unroll
-  for (int d = 0; d < DIM; ++d) r.c[d] = (D)pts[(int64_t)i * DIM + d];
-  r.orig = (int32_t)i;
-  rec[p] = r;
   orig[p] = (int32_t)i;
   fwd[i] = (int32_t)p;
+}
+
+template <typename T, typename D, int DIM>
+__global__ __launch_bounds__(kBlock) void scatter_rows_kernel(const T* __restrict__ pts, int64_t n,
+                                                              const int32_t* __restrict__ fwd,
+                                                              PointRec<D, DIM>* __restrict__ rec) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int32_t p = fwd[i];
+  if (p < 0) return;
+  PointRec<D, DIM> r;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) r.c[d] = (D)pts[i * DIM + d];
+  r.orig = (int32_t)i;
+  rec[p] = r;
 }
 
 unsigned grid_for(int64_t n, int block) {
@@ -401,10 +413,12 @@ void build_impl(pptk_tree* t, const T* dpts, int64_t n, hipStream_t s) {
     // K2: sort (stable: the invalid rows, placed behind the valid ones, stay
     // behind any valid key equal to the sentinel)
     radix_sort_pairs(keys.as<uint64_t>(), vals.as<uint32_t>(), n, DIM * kbits, s);
-    // K3: gather in tree order
-    gather_kernel<T, D, DIM><<<grid_for(n, kBlock), kBlock, 0, s>>>(
-        dpts, vals.as<uint32_t>(), n, nv_dev, t->pts.as<PointRec<D, DIM>>(),
-        t->orig.as<int32_t>(), t->fwd.as<int32_t>());
+    // K3: row maps, then the records in tree order
+    row_maps_kernel<<<grid_for(n, kBlock), kBlock, 0, s>>>(
+        vals.as<uint32_t>(), n, nv_dev, t->orig.as<int32_t>(), t->fwd.as<int32_t>());
+    PK_LAUNCH_CHECK();
+    scatter_rows_kernel<T, D, DIM><<<grid_for(n, kBlock), kBlock, 0, s>>>(
+        dpts, n, t->fwd.as<const int32_t>(), t->pts.as<PointRec<D, DIM>>());
     PK_LAUNCH_CHECK();
   }
   // K3: split level by level until a level creates no children.

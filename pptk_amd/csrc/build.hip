@@ -66,6 +66,18 @@ __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t x) {
   return x;
 }
 
+// Loads the kPer rows first + e * kBlock (e < kPer) of a tile, zero past n.
+template <typename T, int DIM>
+__device__ __forceinline__ void load_tile_rows(const T* __restrict__ pts, int64_t n, int64_t first,
+                                               T (&v)[kPer][DIM]) {
+#pragma unroll
+  for (int e = 0; e < kPer; ++e) {
+    const int64_t i = first + (int64_t)e * kBlock;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) v[e][d] = i < n ? pts[i * DIM + d] : T(0);
+  }
+}
+
 template <typename T, typename D, int DIM>
 __global__ __launch_bounds__(kBlock) void ingest_reduce_kernel(const T* __restrict__ pts, int64_t n,
                                                                uint32_t* __restrict__ blk_cnt,
@@ -77,18 +89,20 @@ __global__ __launch_bounds__(kBlock) void ingest_reduce_kernel(const T* __restri
   D lo[DIM], hi[DIM];
 #pragma unroll
   for (int d = 0; d < DIM; ++d) { lo[d] = pos_inf<D>(); hi[d] = -pos_inf<D>(); }
+  // every row's loads in flight before the first use
+  T v[kPer][DIM];
+  load_tile_rows<T, DIM>(pts, n, first, v);
+#pragma unroll
   for (int e = 0; e < kPer; ++e) {
     const int64_t i = first + (int64_t)e * kBlock;
-    if (i >= n) break;
-    T v[DIM];
-    bool ok = true;
+    bool ok = i < n;
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) { v[d] = pts[i * DIM + d]; ok = ok && elem_finite(v[d]); }
+    for (int d = 0; d < DIM; ++d) ok = ok & elem_finite(v[e][d]);
     if (ok) {
       ++cnt;
 #pragma unroll
       for (int d = 0; d < DIM; ++d) {
-        const D c = (D)v[d];
+        const D c = (D)v[e][d];
         lo[d] = c < lo[d] ? c : lo[d];
         hi[d] = c > hi[d] ? c : hi[d];
       }
@@ -238,14 +252,16 @@ __global__ __launch_bounds__(kBlock) void morton_kernel(const T* __restrict__ pt
   const int w = threadIdx.x >> 6;
   const uint64_t lt_mask = (1ull << lane) - 1ull;
   const int64_t first = (int64_t)blockIdx.x * kTile + threadIdx.x;
+  // the tile's rows stay in registers from the finiteness test to the codes
+  T v[kPer][DIM];
+  load_tile_rows<T, DIM>(pts, n, first, v);
   uint32_t flags = 0;
+#pragma unroll
   for (int e = 0; e < kPer; ++e) {
     const int64_t i = first + (int64_t)e * kBlock;
     bool ok = i < n;
-    if (ok) {
 #pragma unroll
-      for (int d = 0; d < DIM; ++d) ok = ok && elem_finite(pts[i * DIM + d]);
-    }
+    for (int d = 0; d < DIM; ++d) ok = ok & elem_finite(v[e][d]);
     flags |= (uint32_t)ok << e;
     const uint64_t b = __ballot(ok);
     const uint64_t inr = __ballot(i < n);
@@ -275,6 +291,7 @@ __global__ __launch_bounds__(kBlock) void morton_kernel(const T* __restrict__ pt
   const uint32_t tile0 = blk_off[blockIdx.x];
   // invalid rows of the earlier tiles (all full): their rows minus their valid ones
   const uint32_t inv0 = info.n_valid + (uint32_t)((int64_t)blockIdx.x * kTile - tile0);
+#pragma unroll
   for (int e = 0; e < kPer; ++e) {
     const int64_t i = first + (int64_t)e * kBlock;
     const bool ok = (flags >> e) & 1u;
@@ -285,7 +302,7 @@ __global__ __launch_bounds__(kBlock) void morton_kernel(const T* __restrict__ pt
       const uint32_t pos = tile0 + s_off[e * kWaves + w] + (uint32_t)__popcll(b & lt_mask);
       D c[DIM];
 #pragma unroll
-      for (int d = 0; d < DIM; ++d) c[d] = (D)pts[i * DIM + d];
+      for (int d = 0; d < DIM; ++d) c[d] = (D)v[e][d];
       keys[pos] = morton_code<D, DIM>(c, info) >> kshift;
       vals[pos] = (uint32_t)i;
     } else if (bad) {

@@ -38,14 +38,22 @@ __global__ __launch_bounds__(kBlock) void radix_hist_kernel(const uint64_t* __re
   for (int i = tid; i < kWaves * kRadix; i += kBlock) (&cnt[0][0])[i] = 0u;
   __syncthreads();
   const int64_t base = (int64_t)blockIdx.x * kTile;
+  // every key of the tile requested before the first count (kRadix: none)
+  uint32_t dg[kItems];
+  if (base + kTile <= n) {
 #pragma unroll
-  for (int j = 0; j < kItems; ++j) {
-    const int64_t idx = base + (int64_t)j * kBlock + tid;
-    if (idx < n) {
-      const uint32_t d = (uint32_t)(keys[idx] >> shift) & 255u;
-      atomicAdd(&cnt[w][d], 1u);
+    for (int j = 0; j < kItems; ++j)
+      dg[j] = (uint32_t)(keys[base + (int64_t)j * kBlock + tid] >> shift) & 255u;
+  } else {
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+      const int64_t idx = base + (int64_t)j * kBlock + tid;
+      dg[j] = idx < n ? (uint32_t)(keys[idx] >> shift) & 255u : (uint32_t)kRadix;
     }
   }
+#pragma unroll
+  for (int j = 0; j < kItems; ++j)
+    if (dg[j] < (uint32_t)kRadix) atomicAdd(&cnt[w][dg[j]], 1u);
   __syncthreads();
   uint32_t s = 0;
 #pragma unroll
@@ -67,8 +75,6 @@ __global__ __launch_bounds__(kBlock) void radix_scatter_kernel(
   const int lane = tid & 63;
   const int w = tid >> 6;
   for (int i = tid; i < kWaves * kRadix; i += kBlock) (&cnt[0][0])[i] = 0u;
-  gbase[tid] = offs[(int64_t)tid * tiles + blockIdx.x];
-  __syncthreads();
 
   const int64_t wbase = (int64_t)blockIdx.x * kTile + (int64_t)w * (kItems * 64);
   const uint64_t lt_mask = (1ull << lane) - 1ull;
@@ -82,6 +88,9 @@ __global__ __launch_bounds__(kBlock) void radix_scatter_kernel(
     key[j] = ok ? kin[idx] : 0ull;
     val[j] = ok ? vin[idx] : 0u;
   }
+  // the tile's digit offsets are requested after its keys, not waited for first
+  gbase[tid] = offs[(int64_t)tid * tiles + blockIdx.x];
+  __syncthreads();
 #pragma unroll
   for (int j = 0; j < kItems; ++j) {
     const int64_t idx = wbase + (int64_t)j * 64 + lane;

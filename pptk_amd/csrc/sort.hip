@@ -221,10 +221,16 @@ __global__ __launch_bounds__(kScanBlock) void scan_reduce_kernel(const Tin* __re
                                                                  Tout* __restrict__ tile_sums) {
   const int64_t base = (int64_t)blockIdx.x * kScanTile;
   Tout s = 0;
+  if (base + kScanTile <= n) {
+    // a full tile: every load issued before the first add
 #pragma unroll
-  for (int j = 0; j < kScanItems; ++j) {
-    const int64_t i = base + (int64_t)j * kScanBlock + threadIdx.x;
-    if (i < n) s += (Tout)in[i];
+    for (int j = 0; j < kScanItems; ++j) s += (Tout)in[base + (int64_t)j * kScanBlock + threadIdx.x];
+  } else {
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+      const int64_t i = base + (int64_t)j * kScanBlock + threadIdx.x;
+      if (i < n) s += (Tout)in[i];
+    }
   }
   s = wave_sum(s);
   __shared__ Tout ws[kScanBlock / 64];

@@ -75,6 +75,11 @@ __global__ __launch_bounds__(kBlock) void radix_scatter_kernel(
   const int lane = tid & 63;
   const int w = tid >> 6;
   for (int i = tid; i < kWaves * kRadix; i += kBlock) (&cnt[0][0])[i] = 0u;
+  // The digit offsets are read before the keys. Read after them, the scatter
+  // took the same time at 10M keys but 1.23 -> 1.48 ms per pass at 153M keys
+  // (37.5K tiles: a block's 256 offset reads lie 150 KB apart).
+  gbase[tid] = offs[(int64_t)tid * tiles + blockIdx.x];
+  __syncthreads();
 
   const int64_t wbase = (int64_t)blockIdx.x * kTile + (int64_t)w * (kItems * 64);
   const uint64_t lt_mask = (1ull << lane) - 1ull;
@@ -88,9 +93,6 @@ __global__ __launch_bounds__(kBlock) void radix_scatter_kernel(
     key[j] = ok ? kin[idx] : 0ull;
     val[j] = ok ? vin[idx] : 0u;
   }
-  // the tile's digit offsets are requested after its keys, not waited for first
-  gbase[tid] = offs[(int64_t)tid * tiles + blockIdx.x];
-  __syncthreads();
 #pragma unroll
   for (int j = 0; j < kItems; ++j) {
     const int64_t idx = wbase + (int64_t)j * 64 + lane;
